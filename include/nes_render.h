@@ -1,0 +1,173 @@
+/*
+ * nes_render.h -- C-ABI of the MI355X render-and-frame-prep library
+ * (libnes_render.so).
+ *
+ * The reference server (neserver) crosses two boundaries on this path:
+ *   - a process boundary to an external instant-ngp renderer: one
+ *     length-prefixed nesproto::FrameRequest out, one nesproto::RenderedFrame
+ *     back per connection (src/server.cpp:116-212, proto/nes.proto:5-25);
+ *   - the libswscale C ABI for frame prep (sws_getContext / sws_scale /
+ *     sws_freeContext, src/base/video/type_managers.cc:143-155, called from
+ *     include/base/video/rendered_frame.h:24-33).
+ * This header replaces both with in-process calls into hand-written gfx950
+ * kernels.  Plain C: no exceptions cross it, every entry point returns 0 or a
+ * negative errno-style code (mirroring socket_send_blocking's -errno,
+ * src/server.cpp:37).  Callers own host buffers; the library owns device
+ * memory and streams.
+ */
+#ifndef NES_RENDER_H_
+#define NES_RENDER_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ codes */
+#define NES_OK 0
+#define NES_EIO (-5)      /* device / runtime failure */
+#define NES_EAGAIN (-11)  /* not finished yet (poll) */
+#define NES_ENOMEM (-12)  /* device memory or HBM budget exhausted */
+#define NES_EBUSY (-16)   /* queue full */
+#define NES_ENODEV (-19)  /* no usable gfx950 device */
+#define NES_EINVAL (-22)  /* bad argument (odd frame size, null pointer...) */
+#define NES_ERANGE (-34)  /* size beyond a documented limit */
+
+const char *nes_strerror(int code);
+
+/* ----------------------------------------------------------------- camera */
+/* nesproto::Camera (proto/nes.proto:5-10) + FrameRequest.index
+ * (proto/nes.proto:12-16) + the vertical field of view the wire format lacks.
+ * c2w is row-major 3x4 camera-to-world; columns are camera right, image-down,
+ * forward and position (include/base/camera_manager.h:18-19). */
+typedef struct nes_camera {
+  uint64_t index;
+  uint32_t is_left;
+  uint32_t width;
+  uint32_t height;
+  float c2w[12];
+  float fov_y_deg;
+} nes_camera;
+
+/* ------------------------------------------------------------------ scene */
+#define NES_HASH_LEVELS 16
+#define NES_HASH_FEATURES 2
+#define NES_TABLE_ENTRIES 6098925u  /* sum over levels (dense levels 0-4) */
+#define NES_MLP_WEIGHTS 9344u       /* W1 64x32, W2 16x64, W3 64x31, W4 64x64, W5 3x64 */
+#define NES_OCC_WORDS 65536u        /* 128^3 bits */
+#define NES_MAX_STEPS 128
+
+typedef struct nes_scene_desc {
+  const float *table;         /* NES_TABLE_ENTRIES x 2 f32, entry-major */
+  const float *weights;       /* NES_MLP_WEIGHTS f32, row-major [out][in] */
+  const uint32_t *occupancy;  /* NES_OCC_WORDS, bit x+128y+16384z */
+} nes_scene_desc;
+
+/* Fill host buffers with the seeded synthetic scene (any pointer may be NULL).
+ * table_init 0: N(0, 0.1); 1: U(-1e-4, 1e-4). */
+int nes_scene_synthesize(uint64_t seed, int table_init, float *table,
+                         float *weights, uint32_t *occupancy);
+/* Camera on a radius-r orbit around the cube centre, looking at it; the eye
+ * is displaced by eye_offset along the camera right axis. */
+int nes_orbit_camera(double azimuth_deg, double elevation_deg, double radius,
+                     double eye_offset, float c2w[12]);
+
+/* ---------------------------------------------------------------- context */
+#define NES_PRECISION_FP32 0   /* FP32 table, FP32 VALU MLPs */
+#define NES_PRECISION_MIXED 1  /* FP16 table, FP16 MFMA MLPs (FP32 accum) */
+
+typedef struct nes_ctx nes_ctx;
+
+/* Uploads the scene to the device.  hbm_budget_bytes bounds the context's
+ * work buffers (0 = 16 GiB). Replaces connecting to a renderer
+ * (socket_manage_thread, src/server.cpp:216-283). */
+int nes_ctx_create(const nes_scene_desc *scene, uint64_t hbm_budget_bytes,
+                   nes_ctx **out);
+int nes_ctx_destroy(nes_ctx *ctx);
+/* Replace the scene parameters (e.g. after training). */
+int nes_ctx_update_scene(nes_ctx *ctx, const nes_scene_desc *scene);
+
+/* --------------------------------------------------------------- render */
+/* One view's host outputs; any pointer may be NULL.  rgb24 is tightly packed
+ * W*H*3, top row first (as RenderedFrame.frame, src/base/video/
+ * rendered_frame.cc:14-18); depth is W*H f32 expected depth; rgb_f32 is the
+ * unquantised colour W*H*3; n_samples W*H evaluated samples per pixel. */
+typedef struct nes_view_out {
+  uint8_t *rgb24;
+  float *depth;
+  float *rgb_f32;
+  uint16_t *n_samples;
+} nes_view_out;
+
+typedef struct nes_render_stats {
+  uint64_t rays;
+  uint64_t samples;         /* evaluated network samples */
+  double ms_total;          /* device time of the whole render */
+  double ms_eval;           /* device time of the network kernel */
+  double ms_march;
+  double ms_composite;
+} nes_render_stats;
+
+/* Synchronous render of n views (e.g. a stereo pair) in one launch sequence:
+ * the in-process replacement of one FrameRequest -> RenderedFrame round trip
+ * (src/server.cpp:140-194).  steps_per_pass 0 = all steps in one pass; smaller
+ * values march in passes and drop rays once T < 1e-4 (results are identical). */
+int nes_render(nes_ctx *ctx, int precision, const nes_camera *views, int n_views,
+               const nes_view_out *outs, int steps_per_pass,
+               nes_render_stats *stats);
+
+/* Device-resident variant for benchmarks and chaining: outputs stay in the
+ * context's device buffers; nes_render_fetch copies view i to the host. */
+int nes_render_device(nes_ctx *ctx, int precision, const nes_camera *views,
+                      int n_views, int steps_per_pass, nes_render_stats *stats);
+int nes_render_fetch(nes_ctx *ctx, int view, const nes_view_out *out);
+
+/* ----------------------------------------------------------- frame prep */
+#define NES_SWS_X86 0  /* libswscale 5.5 on x86-64, flags 0 (deployed path) */
+#define NES_SWS_C 1    /* libswscale 5.5 portable C output paths */
+
+/* Nearer-depth composite of two renderer outputs then RGB24 -> YUV420P with
+ * the arithmetic of sws_scale(flags 0) (src/base/video/type_managers.cc:
+ * 143-155).  Host pointers.  rgb_b/depth_b may be NULL (plain conversion of
+ * A, which is what RenderedFrame::convert_frame does,
+ * include/base/video/rendered_frame.h:24-33).  w and h must be even
+ * (src/base/camera_manager.cc:38-44).  A pixel takes B when depth_b is nearer
+ * (NaN counts as +inf; ties keep A). */
+int nes_frameprep(const uint8_t *rgb_a, const float *depth_a,
+                  const uint8_t *rgb_b, const float *depth_b, int w, int h,
+                  int rgb_stride, uint8_t *y, int y_stride, uint8_t *u,
+                  int u_stride, uint8_t *v, int v_stride, int sws_mode);
+
+/* Same on device pointers, n_frames frames laid out frame_bytes apart in
+ * each buffer (rgb: w*h*3, depth: w*h*4, planes: y_stride*h etc.); runs on the
+ * library stream and returns after completion.  elapsed_ms (optional)
+ * receives the kernel time measured with device events. */
+int nes_frameprep_device(const uint8_t *rgb_a, const float *depth_a,
+                         const uint8_t *rgb_b, const float *depth_b, int w,
+                         int h, int n_frames, uint8_t *y, int y_stride,
+                         uint8_t *u, int u_stride, uint8_t *v, int v_stride,
+                         int sws_mode, float *elapsed_ms);
+
+/* GRAY8 (full range) -> YUV420P, the depth stream of
+ * RenderedFrame::convert_frame (include/base/video/rendered_frame.h:30-31). */
+int nes_gray8_to_yuv420p(const uint8_t *gray, int w, int h, int g_stride,
+                         uint8_t *y, int y_stride, uint8_t *u, int u_stride,
+                         uint8_t *v, int v_stride, int sws_mode);
+
+/* ------------------------------------------------------ device memory */
+/* Minimal device-memory helpers so callers (tests, benchmarks) need no other
+ * GPU runtime. */
+int nes_device_alloc(size_t bytes, void **ptr);
+int nes_device_free(void *ptr);
+int nes_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int nes_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int nes_device_info(char *name, size_t name_len, int *cu_count,
+                    uint64_t *hbm_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NES_RENDER_H_ */

@@ -1,0 +1,140 @@
+// common.hip -- error reporting, device selection, scratch buffers and the
+// device-memory helpers of the C-ABI.
+#include <cstring>
+
+#include "nes_internal.h"
+
+namespace nes {
+
+namespace {
+std::mutex g_err_mu;
+std::string g_last_error;
+std::once_flag g_dev_once;
+int g_dev_status = NES_ENODEV;
+hipStream_t g_stream = nullptr;
+}  // namespace
+
+void set_last_error(const std::string &msg) {
+  std::lock_guard<std::mutex> lk(g_err_mu);
+  g_last_error = msg;
+}
+
+int ensure_device() {
+  std::call_once(g_dev_once, [] {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1) {
+      set_last_error("no HIP device visible");
+      g_dev_status = NES_ENODEV;
+      return;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, 0) != hipSuccess) {
+      g_dev_status = NES_EIO;
+      return;
+    }
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+      set_last_error(std::string("device is ") + prop.gcnArchName +
+                     ", this library is built for gfx950 only");
+      g_dev_status = NES_ENODEV;
+      return;
+    }
+    if (hipSetDevice(0) != hipSuccess ||
+        hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking) != hipSuccess) {
+      g_dev_status = NES_EIO;
+      return;
+    }
+    g_dev_status = NES_OK;
+  });
+  if (g_dev_status == NES_OK) hipSetDevice(0);
+  return g_dev_status;
+}
+
+hipStream_t library_stream() { return g_stream; }
+
+int DevBuf::reserve(size_t need) {
+  if (need <= bytes) return NES_OK;
+  release();
+  size_t cap = need + need / 8 + 256;
+  hipError_t e = hipMalloc(&ptr, cap);
+  if (e != hipSuccess) {
+    ptr = nullptr;
+    bytes = 0;
+    set_last_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+    return NES_ENOMEM;
+  }
+  bytes = cap;
+  return NES_OK;
+}
+
+void DevBuf::release() {
+  if (ptr) hipFree(ptr);
+  ptr = nullptr;
+  bytes = 0;
+}
+
+}  // namespace nes
+
+extern "C" {
+
+const char *nes_strerror(int code) {
+  switch (code) {
+    case NES_OK: return "ok";
+    case NES_EIO: return "device or runtime failure";
+    case NES_EAGAIN: return "not finished";
+    case NES_ENOMEM: return "out of device memory or HBM budget";
+    case NES_EBUSY: return "queue full";
+    case NES_ENODEV: return "no usable gfx950 device";
+    case NES_EINVAL: return "invalid argument";
+    case NES_ERANGE: return "size beyond limit";
+    default: return "unknown error";
+  }
+}
+
+// Last detailed error message (thread-shared); for diagnostics only.
+const char *nes_last_error(void) {
+  static thread_local std::string copy;
+  std::lock_guard<std::mutex> lk(nes::g_err_mu);
+  copy = nes::g_last_error;
+  return copy.c_str();
+}
+
+int nes_device_alloc(size_t bytes, void **ptr) {
+  if (!ptr) return NES_EINVAL;
+  NES_TRY(nes::ensure_device());
+  NES_HIP_TRY(hipMalloc(ptr, bytes ? bytes : 1));
+  return NES_OK;
+}
+
+int nes_device_free(void *ptr) {
+  NES_TRY(nes::ensure_device());
+  if (ptr) NES_HIP_TRY(hipFree(ptr));
+  return NES_OK;
+}
+
+int nes_memcpy_h2d(void *dst, const void *src, size_t bytes) {
+  NES_TRY(nes::ensure_device());
+  NES_HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return NES_OK;
+}
+
+int nes_memcpy_d2h(void *dst, const void *src, size_t bytes) {
+  NES_TRY(nes::ensure_device());
+  NES_HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return NES_OK;
+}
+
+int nes_device_info(char *name, size_t name_len, int *cu_count,
+                    uint64_t *hbm_bytes) {
+  NES_TRY(nes::ensure_device());
+  hipDeviceProp_t prop;
+  NES_HIP_TRY(hipGetDeviceProperties(&prop, 0));
+  if (name && name_len) {
+    std::strncpy(name, prop.gcnArchName, name_len - 1);
+    name[name_len - 1] = 0;
+  }
+  if (cu_count) *cu_count = prop.multiProcessorCount;
+  if (hbm_bytes) *hbm_bytes = prop.totalGlobalMem;
+  return NES_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,80 @@
+// render.h -- device data structures of the render pipeline and the context.
+#pragma once
+
+#include "nes_internal.h"
+
+namespace nes {
+
+// One camera of a render batch.  Ray slots of a view are 8x8 pixel tiles laid
+// out tile-major so that a wavefront's 64 rays are a compact screen block.
+struct ViewDesc {
+  float c2w[12];
+  float focal;
+  uint32_t width, height, tiles_x;
+  uint32_t slot_base, slot_count;  // global ray-slot range of the view
+  uint32_t px_base;                // offset of the view in the output arrays
+  uint32_t pad;
+};
+
+struct RayData {  // 32 bytes
+  float o[3];
+  float d[3];
+  float t_start;
+  uint32_t n_steps;  // candidate steps inside the box (0 = miss / no pixel)
+};
+
+struct RayAcc {  // 32 bytes, compositing state carried across passes
+  float T, r, g, b, depth;
+  uint32_t n_samples;
+  uint32_t done;
+  uint32_t pad;
+};
+
+// Sample record: ray slot in the high 25 bits, step index in the low 7.
+constexpr uint32_t kMaxSlots = 1u << 25;
+
+struct RenderWork {
+  DevBuf views, rays, acc, counts, offsets, samples, sample_out, out, total;
+  uint32_t *total_host = nullptr;  // pinned
+  hipEvent_t ev[8] = {};
+  uint32_t n_views = 0, n_slots = 0;
+  size_t out_px = 0;
+  std::vector<ViewDesc> host_views;
+};
+
+struct SceneDev {
+  float *table_f32 = nullptr;      // entries x 2
+  _Float16 *table_f16 = nullptr;   // entries x 2
+  float *weights_f32 = nullptr;    // 9344
+  _Float16 *wfrag = nullptr;       // 20 MFMA A-fragments x 64 lanes x 8
+  uint32_t *occ = nullptr;         // 65536
+};
+
+int scene_upload(SceneDev &s, const nes_scene_desc *desc);
+void scene_free(SceneDev &s);
+// Permute the MLP weights into the A-operand fragments of the 20
+// v_mfma_f32_16x16x32_f16 instructions one 16-sample tile uses.
+void build_mlp_fragments(const float *w, _Float16 *frag);
+
+struct RenderOutputs {  // byte offsets inside RenderWork::out per pixel
+  static size_t bytes_per_px() { return 3 + 4 + 12 + 2; }
+};
+
+// Enqueue a full render of views on stream; returns after the sample count
+// read-backs (one host sync per pass).  Outputs land in work.out.
+int render_enqueue(const SceneDev &scene, RenderWork &work, int precision,
+                   const nes_camera *views, int n_views, int steps_per_pass,
+                   hipStream_t stream, nes_render_stats *stats);
+int render_fetch(RenderWork &work, int view, const nes_view_out *out,
+                 hipStream_t stream);
+void render_work_free(RenderWork &w);
+
+}  // namespace nes
+
+struct nes_ctx {
+  nes::SceneDev scene;
+  nes::RenderWork work;
+  hipStream_t stream = nullptr;
+  uint64_t budget = 0;
+  std::mutex mu;
+};

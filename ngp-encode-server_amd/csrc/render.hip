@@ -1,0 +1,835 @@
+// render.hip -- the render path the reference delegates to its external
+// instant-ngp renderer (src/server.cpp:161-194 sends a FrameRequest and waits
+// for RGB + depth): ray generation, occupancy-grid march, 16-level hash-grid
+// encoding, density/colour MLPs and volume compositing.
+//
+// Pipeline per pass (a pass covers a window of march steps; one pass of 128
+// steps is the default):
+//   march_count  one thread per ray slot: camera ray, box clip, count
+//                occupied steps; wave-aggregated atomic for the pass total
+//   march_fill   same walk, writes 4-byte sample records (slot << 7 | step)
+//                into per-ray contiguous segments (wave scan + one atomic)
+//   eval_*       the network over the dense sample list:
+//                  eval_fp32  FP32 table, per-lane FP32 VALU MLP
+//                  eval_mixed FP16 table, FP16 MFMA MLP with FP32 accumulate
+//   composite    one thread per ray: front-to-back alpha compositing in FP32,
+//                stops at T < 1e-4, writes RGB24 / depth / float RGB
+// Both precisions share every kernel except eval_*, so their throughput ratio
+// isolates the encoding + MLP arithmetic.
+//
+// Sample geometry (ray, box clip, step positions, occupancy cell) is defined
+// in FP32 with explicit fused multiply-adds and correctly rounded division and
+// square root, so every implementation visits the same samples.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "render.h"
+
+namespace nes {
+
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef _Float16 half4 __attribute__((ext_vector_type(4)));
+typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+typedef float float4v __attribute__((ext_vector_type(4)));
+
+// ------------------------------------------------------------- geometry
+__device__ __forceinline__ void make_ray(const ViewDesc &v, uint32_t px,
+                                         uint32_t py, float o[3], float d[3]) {
+#pragma clang fp contract(off)
+  const float u = (((float)px + 0.5f) - 0.5f * (float)v.width) / v.focal;
+  const float w = (((float)py + 0.5f) - 0.5f * (float)v.height) / v.focal;
+  float dw[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    dw[i] = fmaf(v.c2w[4 * i + 0], u, fmaf(v.c2w[4 * i + 1], w, v.c2w[4 * i + 2]));
+  const float n = sqrtf(fmaf(dw[0], dw[0], fmaf(dw[1], dw[1], dw[2] * dw[2])));
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    d[i] = dw[i] / n;
+    o[i] = v.c2w[4 * i + 3];
+  }
+}
+
+__device__ __forceinline__ float step_t(int k, float t_start) {
+  return fmaf((float)k + 0.5f, kDelta, t_start);
+}
+
+// Returns the number of candidate steps (<= 128) inside [0,1]^3.
+__device__ __forceinline__ uint32_t ray_box(const float o[3], const float d[3],
+                                            float *t_start) {
+#pragma clang fp contract(off)
+  float tn = -INFINITY, tf = INFINITY;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if (d[a] == 0.0f) {
+      if (o[a] < 0.0f || o[a] > 1.0f) return 0;
+      continue;
+    }
+    const float t0 = (0.0f - o[a]) / d[a];
+    const float t1 = (1.0f - o[a]) / d[a];
+    const float lo = t0 < t1 ? t0 : t1;
+    const float hi = t0 < t1 ? t1 : t0;
+    tn = lo > tn ? lo : tn;
+    tf = hi < tf ? hi : tf;
+  }
+  const float ts = tn > 0.0f ? tn : 0.0f;
+  *t_start = ts;
+  if (!(tf > ts)) return 0;
+  uint32_t n = 0;
+  while (n < (uint32_t)kMaxSteps && step_t((int)n, ts) < tf) ++n;
+  return n;
+}
+
+__device__ __forceinline__ bool occupied(const uint32_t *__restrict__ occ,
+                                         const float x[3]) {
+  int c[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int ci = (int)floorf(x[a] * 128.0f);
+    c[a] = min(max(ci, 0), kOccRes - 1);
+  }
+  const uint32_t cell = (uint32_t)(c[0] + kOccRes * (c[1] + kOccRes * c[2]));
+  return (__ldg(occ + (cell >> 5)) >> (cell & 31)) & 1u;
+}
+
+__device__ __forceinline__ void step_pos(const RayData &r, int k, float x[3],
+                                         float *t) {
+  const float tt = step_t(k, r.t_start);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) x[a] = fmaf(tt, r.d[a], r.o[a]);
+  *t = tt;
+}
+
+__device__ __forceinline__ RayData load_ray(const RayData *rays, uint32_t slot) {
+  const float4 *p = reinterpret_cast<const float4 *>(rays + slot);
+  const float4 a = p[0], b = p[1];
+  RayData r;
+  r.o[0] = a.x; r.o[1] = a.y; r.o[2] = a.z;
+  r.d[0] = a.w; r.d[1] = b.x; r.d[2] = b.y;
+  r.t_start = b.z;
+  r.n_steps = __float_as_uint(b.w);
+  return r;
+}
+
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// ----------------------------------------------------------- march kernels
+__global__ __launch_bounds__(256) void march_count(
+    const ViewDesc *__restrict__ views, uint32_t n_views, uint32_t n_slots,
+    const uint32_t *__restrict__ occ, RayData *rays, RayAcc *acc,
+    uint32_t *counts, uint32_t *total, int k_begin, int k_end, int first) {
+  const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t cnt = 0;
+  if (slot < n_slots) {
+    RayData r;
+    bool active;
+    if (first) {
+      uint32_t vi = 0;
+      for (uint32_t i = 1; i < n_views; ++i)
+        if (slot >= views[i].slot_base) vi = i;
+      const ViewDesc &v = views[vi];
+      const uint32_t local = slot - v.slot_base;
+      const uint32_t tile = local >> 6, within = local & 63;
+      const uint32_t px = (tile % v.tiles_x) * 8 + (within & 7);
+      const uint32_t py = (tile / v.tiles_x) * 8 + (within >> 3);
+      RayAcc a;
+      a.T = 1.0f; a.r = a.g = a.b = 0.0f; a.depth = 0.0f; a.n_samples = 0;
+      if (px < v.width && py < v.height) {
+        make_ray(v, px, py, r.o, r.d);
+        r.n_steps = ray_box(r.o, r.d, &r.t_start);
+        a.pad = v.px_base + py * v.width + px;
+      } else {
+        r.o[0] = r.o[1] = r.o[2] = 0.0f;
+        r.d[0] = r.d[1] = r.d[2] = 0.0f;
+        r.t_start = 0.0f;
+        r.n_steps = 0;
+        a.pad = 0xffffffffu;
+      }
+      a.done = r.n_steps == 0;
+      rays[slot] = r;
+      acc[slot] = a;
+      active = !a.done;
+    } else {
+      r = load_ray(rays, slot);
+      active = !acc[slot].done;
+    }
+    if (active) {
+      const int kend = min(k_end, (int)r.n_steps);
+      for (int k = k_begin; k < kend; ++k) {
+        float x[3], t;
+        step_pos(r, k, x, &t);
+        cnt += occupied(occ, x) ? 1u : 0u;
+      }
+    }
+    counts[slot] = cnt;
+  }
+  const uint32_t s = wave_sum(cnt);
+  if (lane_id() == 0 && s) atomicAdd(total, s);
+}
+
+__global__ __launch_bounds__(256) void march_fill(
+    uint32_t n_slots, const uint32_t *__restrict__ occ,
+    const RayData *__restrict__ rays, const uint32_t *__restrict__ counts,
+    uint32_t *offsets, uint32_t *samples, uint32_t *cursor, int k_begin,
+    int k_end) {
+  const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = lane_id();
+  const uint32_t cnt = slot < n_slots ? counts[slot] : 0u;
+  uint32_t incl = cnt;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t t = __shfl_up(incl, o);
+    if (lane >= o) incl += t;
+  }
+  const uint32_t wave_total = __shfl(incl, 63);
+  uint32_t base = 0;
+  if (lane == 0 && wave_total) base = atomicAdd(cursor, wave_total);
+  base = __shfl(base, 0);
+  if (slot >= n_slots) return;
+  uint32_t off = base + incl - cnt;
+  offsets[slot] = off;
+  if (!cnt) return;
+  const RayData r = load_ray(rays, slot);
+  const int kend = min(k_end, (int)r.n_steps);
+  for (int k = k_begin; k < kend; ++k) {
+    float x[3], t;
+    step_pos(r, k, x, &t);
+    if (occupied(occ, x)) samples[off++] = (slot << 7) | (uint32_t)k;
+  }
+}
+
+// --------------------------------------------------------------- encoding
+template <int L>
+__device__ __forceinline__ uint32_t level_entry(uint32_t x, uint32_t y, uint32_t z) {
+  if constexpr (L < kDenseLevels) {
+    constexpr uint32_t n1 = (uint32_t)kLevelRes[L] + 1u;
+    return kLevelOffset[L] + x + y * n1 + z * (n1 * n1);
+  } else {
+    return kLevelOffset[L] + ((x ^ (y * kPrime1) ^ (z * kPrime2)) & ((1u << kLog2T) - 1u));
+  }
+}
+
+__device__ __forceinline__ float2 load_feat(const float *__restrict__ t, uint32_t e) {
+  return __ldg(reinterpret_cast<const float2 *>(t) + e);
+}
+__device__ __forceinline__ float2 load_feat(const _Float16 *__restrict__ t, uint32_t e) {
+  const uint32_t bits = __ldg(reinterpret_cast<const uint32_t *>(t) + e);
+  half2v h = __builtin_bit_cast(half2v, bits);
+  return make_float2((float)h.x, (float)h.y);
+}
+
+template <int L, class Tab>
+__device__ __forceinline__ void encode_level(const Tab *__restrict__ table,
+                                             const float xc[3], float &f0,
+                                             float &f1) {
+  constexpr float scale = (float)kLevelRes[L];
+  uint32_t i0[3];
+  float fr[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float p = xc[a] * scale;
+    const float fl = fminf(floorf(p), scale - 1.0f);
+    i0[a] = (uint32_t)fl;
+    fr[a] = p - fl;
+  }
+  float2 v[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c)
+    v[c] = load_feat(table, level_entry<L>(i0[0] + (c & 1), i0[1] + ((c >> 1) & 1),
+                                           i0[2] + ((c >> 2) & 1)));
+  const float gx = 1.0f - fr[0], gy = 1.0f - fr[1], gz = 1.0f - fr[2];
+  float a0 = 0.0f, a1 = 0.0f;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const float w = ((c & 1) ? fr[0] : gx) * (((c >> 1) & 1) ? fr[1] : gy) *
+                    (((c >> 2) & 1) ? fr[2] : gz);
+    a0 = fmaf(w, v[c].x, a0);
+    a1 = fmaf(w, v[c].y, a1);
+  }
+  f0 = a0;
+  f1 = a1;
+}
+
+template <class Tab, int L = 0>
+__device__ __forceinline__ void encode(const Tab *__restrict__ table,
+                                       const float xc[3], float feat[32]) {
+  if constexpr (L < kLevels) {
+    encode_level<L>(table, xc, feat[2 * L], feat[2 * L + 1]);
+    encode<Tab, L + 1>(table, xc, feat);
+  }
+}
+
+__device__ __forceinline__ void sh4(const float d[3], float sh[16]) {
+  const float x = d[0], y = d[1], z = d[2];
+  const float xx = x * x, yy = y * y, zz = z * z;
+  sh[0] = 0.28209479177387814f;
+  sh[1] = -0.48860251190291987f * y;
+  sh[2] = 0.48860251190291987f * z;
+  sh[3] = -0.48860251190291987f * x;
+  sh[4] = 1.0925484305920792f * x * y;
+  sh[5] = -1.0925484305920792f * y * z;
+  sh[6] = 0.94617469575755997f * zz - 0.31539156525251999f;
+  sh[7] = -1.0925484305920792f * x * z;
+  sh[8] = 0.54627421529603959f * xx - 0.54627421529603959f * yy;
+  sh[9] = 0.59004358992664352f * y * (-3.0f * xx + yy);
+  sh[10] = 2.8906114426405538f * x * y * z;
+  sh[11] = 0.45704579946446572f * y * (1.0f - 5.0f * zz);
+  sh[12] = 0.3731763325901154f * z * (5.0f * zz - 3.0f);
+  sh[13] = 0.45704579946446572f * x * (1.0f - 5.0f * zz);
+  sh[14] = 1.4453057213202769f * z * (xx - yy);
+  sh[15] = 0.59004358992664352f * x * (-xx + 3.0f * yy);
+}
+
+__device__ __forceinline__ void sample_geometry(const uint32_t *__restrict__ samples,
+                                                const RayData *__restrict__ rays,
+                                                uint32_t i, float xc[3], float d[3]) {
+  const uint32_t rec = samples[i];
+  const RayData r = load_ray(rays, rec >> 7);
+  float x[3], t;
+  step_pos(r, (int)(rec & 127u), x, &t);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    xc[a] = fminf(fmaxf(x[a], 0.0f), 1.0f);
+    d[a] = r.d[a];
+  }
+}
+
+__device__ __forceinline__ float sigmoidf_(float v) { return 1.0f / (1.0f + expf(-v)); }
+
+// ------------------------------------------------------------ FP32 eval
+template <int NI, int NO, bool kRelu>
+__device__ __forceinline__ void dense_f32(const float *w, const float *in, float *out) {
+#pragma unroll
+  for (int o = 0; o < NO; ++o) {
+    float acc = 0.0f;
+#pragma unroll
+    for (int i = 0; i < NI; ++i) acc = fmaf(w[o * NI + i], in[i], acc);
+    out[o] = kRelu ? fmaxf(acc, 0.0f) : acc;
+    // keep the scheduler from hoisting every weight read of the layer
+    if ((o & 3) == 3) __asm__ volatile("" ::: "memory");
+  }
+}
+
+// One lane per sample; the 9344 weights sit in LDS and are read as
+// wave-uniform broadcasts.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void eval_fp32(
+    const uint32_t *__restrict__ samples, uint32_t n,
+    const RayData *__restrict__ rays, const float *__restrict__ table,
+    const float *__restrict__ Wg, float4 *__restrict__ out) {
+  __shared__ __attribute__((aligned(16))) float W[kWeights];
+  for (int i = (int)threadIdx.x; i < kWeights / 4; i += (int)blockDim.x)
+    reinterpret_cast<float4 *>(W)[i] = reinterpret_cast<const float4 *>(Wg)[i];
+  __syncthreads();
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += gridDim.x * blockDim.x) {
+    float xc[3], d[3];
+    sample_geometry(samples, rays, i, xc, d);
+    float feat[32];
+    encode(table, xc, feat);
+    float h1[64], o2[16], in3[31], h3[64], h4[64], o5[3], sh[16];
+    dense_f32<32, 64, true>(W + kW1, feat, h1);
+    dense_f32<64, 16, false>(W + kW2, h1, o2);
+    sh4(d, sh);
+#pragma unroll
+    for (int j = 0; j < 16; ++j) in3[j] = sh[j];
+#pragma unroll
+    for (int j = 0; j < 15; ++j) in3[16 + j] = o2[1 + j];
+    dense_f32<31, 64, true>(W + kW3, in3, h3);
+    dense_f32<64, 64, true>(W + kW4, h3, h4);
+    dense_f32<64, 3, false>(W + kW5, h4, o5);
+    out[i] = make_float4(expf(o2[0]), sigmoidf_(o5[0]), sigmoidf_(o5[1]), sigmoidf_(o5[2]));
+  }
+}
+
+// ----------------------------------------------------------- mixed eval
+// MFMA tile: v_mfma_f32_16x16x32_f16, D[16x16] += A[16x32] B[32x16].
+// Rows (M) are output neurons, columns (N) are 16 samples, K is the input.
+// Lane l holds B column l%16 at K-slots (l/16, s = 0..7); D rows 4(l/16)+r.
+// A neuron block's D therefore already sits where the next layer's B needs it
+// once the K index of slot (g, s) is taken to be
+//   s < 4 : block 2c   neuron 4g + s        s >= 4 : block 2c+1 neuron 4g + s-4
+// which the host bakes into the weight fragments (build_mlp_fragments).
+constexpr int kEvalWaves = 4;
+constexpr int kLdsRow = 48;  // 32 features + 16 SH (halves)
+
+__device__ __forceinline__ float4v mfma(half8 a, half8 b, float4v c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ half8 pack_relu(float4v lo, float4v hi) {
+  half8 r;
+  r[0] = (_Float16)fmaxf(lo[0], 0.0f); r[1] = (_Float16)fmaxf(lo[1], 0.0f);
+  r[2] = (_Float16)fmaxf(lo[2], 0.0f); r[3] = (_Float16)fmaxf(lo[3], 0.0f);
+  r[4] = (_Float16)fmaxf(hi[0], 0.0f); r[5] = (_Float16)fmaxf(hi[1], 0.0f);
+  r[6] = (_Float16)fmaxf(hi[2], 0.0f); r[7] = (_Float16)fmaxf(hi[3], 0.0f);
+  return r;
+}
+
+__device__ __forceinline__ void wave_lds_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <class Tab, int L = 0>
+__device__ __forceinline__ void encode_to_lds(const Tab *__restrict__ table,
+                                              const float xc[3], _Float16 *row) {
+  if constexpr (L < kLevels) {
+    float f0, f1;
+    encode_level<L>(table, xc, f0, f1);
+    half2v h;
+    h.x = (_Float16)f0;
+    h.y = (_Float16)f1;
+    *reinterpret_cast<half2v *>(row + 2 * L) = h;
+    encode_to_lds<Tab, L + 1>(table, xc, row);
+  }
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void eval_mixed(
+    const uint32_t *__restrict__ samples, uint32_t n,
+    const RayData *__restrict__ rays, const _Float16 *__restrict__ table,
+    const half8 *__restrict__ wfrag, float4 *__restrict__ out) {
+  __shared__ __attribute__((aligned(16))) _Float16 lds[kEvalWaves][64 * kLdsRow];
+  __shared__ half8 wl[20 * 64];
+  for (int i = (int)threadIdx.x; i < 20 * 64; i += (int)blockDim.x) wl[i] = wfrag[i];
+  __syncthreads();
+  const int lane = lane_id();
+  const int wave = (int)(threadIdx.x >> 6);
+  _Float16 *row_base = lds[wave];
+  const half8 *wf = wl + lane;  // fragment m at wf[64 * m]
+  const int j = lane & 15, g = lane >> 4;
+  const float4v zero = {0.0f, 0.0f, 0.0f, 0.0f};
+
+  const uint32_t n_chunks = (n + 63) / 64;
+  for (uint32_t chunk = blockIdx.x * kEvalWaves + wave; chunk < n_chunks;
+       chunk += gridDim.x * kEvalWaves) {
+    const uint32_t i = chunk * 64 + (uint32_t)lane;
+    // Encode this lane's sample into its LDS row.
+    _Float16 *row = row_base + lane * kLdsRow;
+    if (i < n) {
+      float xc[3], d[3], sh[16];
+      sample_geometry(samples, rays, i, xc, d);
+      encode_to_lds(table, xc, row);
+      sh4(d, sh);
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        half8 h;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) h[e] = (_Float16)sh[8 * q + e];
+        *reinterpret_cast<half8 *>(row + 32 + 8 * q) = h;
+      }
+    } else {
+      const half8 hz = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+      for (int q = 0; q < 6; ++q) *reinterpret_cast<half8 *>(row + 8 * q) = hz;
+    }
+    wave_lds_fence();
+#pragma unroll 1
+    for (int t = 0; t < 4; ++t) {
+      const _Float16 *src = row_base + (16 * t + j) * kLdsRow;
+      const half8 b1 = *reinterpret_cast<const half8 *>(src + 8 * g);
+      float4v d1[4];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) d1[m] = mfma(wf[64 * m], b1, zero);
+      float4v d2 = mfma(wf[64 * 4], pack_relu(d1[0], d1[1]), zero);
+      d2 = mfma(wf[64 * 5], pack_relu(d1[2], d1[3]), d2);
+      const half4 shv = *reinterpret_cast<const half4 *>(src + 32 + 4 * g);
+      half8 b3;
+      b3[0] = (_Float16)d2[0]; b3[1] = (_Float16)d2[1];
+      b3[2] = (_Float16)d2[2]; b3[3] = (_Float16)d2[3];
+      b3[4] = shv[0]; b3[5] = shv[1]; b3[6] = shv[2]; b3[7] = shv[3];
+      float4v d3[4];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) d3[m] = mfma(wf[64 * (6 + m)], b3, zero);
+      const half8 b4a = pack_relu(d3[0], d3[1]), b4b = pack_relu(d3[2], d3[3]);
+      float4v d4[4];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        d4[m] = mfma(wf[64 * (10 + 2 * m)], b4a, zero);
+        d4[m] = mfma(wf[64 * (11 + 2 * m)], b4b, d4[m]);
+      }
+      float4v d5 = mfma(wf[64 * 18], pack_relu(d4[0], d4[1]), zero);
+      d5 = mfma(wf[64 * 19], pack_relu(d4[2], d4[3]), d5);
+      const uint32_t s = chunk * 64 + 16 * t + j;
+      if (g == 0 && s < n)
+        out[s] = make_float4(expf(d2[0]), sigmoidf_(d5[0]), sigmoidf_(d5[1]), sigmoidf_(d5[2]));
+    }
+    wave_lds_fence();
+  }
+}
+
+// ------------------------------------------------------------- composite
+struct OutPtrs {
+  uint8_t *rgb24;
+  float *depth;
+  float *rgbf;
+  uint16_t *nsamp;
+};
+
+__global__ __launch_bounds__(256) void composite(
+    uint32_t n_slots, const RayData *__restrict__ rays, RayAcc *acc,
+    const uint32_t *__restrict__ counts, const uint32_t *__restrict__ offsets,
+    const uint32_t *__restrict__ samples, const float4 *__restrict__ sout,
+    int last, OutPtrs o) {
+  const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= n_slots) return;
+  RayAcc a = acc[slot];
+  const uint32_t cnt = counts[slot];
+  if (!a.done && cnt) {
+    const uint32_t off = offsets[slot];
+    const float ts = load_ray(rays, slot).t_start;
+    for (uint32_t j = 0; j < cnt; ++j) {
+      const float4 s = sout[off + j];
+      const float t = step_t((int)(samples[off + j] & 127u), ts);
+      const float sd = s.x * kDelta;
+      const float alpha = -expm1f(-sd);
+      const float w = a.T * alpha;
+      a.r = fmaf(w, s.y, a.r);
+      a.g = fmaf(w, s.z, a.g);
+      a.b = fmaf(w, s.w, a.b);
+      a.depth = fmaf(w, t, a.depth);
+      a.T *= expf(-sd);
+      a.n_samples++;
+      if (a.T < kTStop) {
+        a.done = 1;
+        break;
+      }
+    }
+    acc[slot] = a;
+  }
+  if (last && a.pad != 0xffffffffu) {
+    const uint32_t p = a.pad;
+    const float c[3] = {a.r, a.g, a.b};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const float q = floorf(fmaf(c[k], 255.0f, 0.5f));
+      o.rgb24[3 * (size_t)p + k] = (uint8_t)fminf(fmaxf(q, 0.0f), 255.0f);
+      o.rgbf[3 * (size_t)p + k] = c[k];
+    }
+    o.depth[p] = a.depth;
+    o.nsamp[p] = (uint16_t)a.n_samples;
+  }
+}
+
+__global__ void zero_u32(uint32_t *p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+// ----------------------------------------------------------------- host
+void build_mlp_fragments(const float *w, _Float16 *frag) {
+  // slot (g, s) of K-chunk c -> input neuron of a 64-wide hidden layer
+  auto hid = [](int c, int g, int s) { return s < 4 ? 32 * c + 4 * g + s : 32 * c + 16 + 4 * g + (s - 4); };
+  auto put = [&](int mf, int lane, int s, float v) { frag[((size_t)mf * 64 + lane) * 8 + s] = (_Float16)v; };
+  for (int lane = 0; lane < 64; ++lane) {
+    const int i = lane & 15, g = lane >> 4;
+    for (int s = 0; s < 8; ++s) {
+      // layer 1: 32 features (natural K order) -> 64
+      for (int m = 0; m < 4; ++m) put(m, lane, s, w[kW1 + (16 * m + i) * 32 + 8 * g + s]);
+      // layer 2: 64 -> 16, two K-chunks
+      for (int c = 0; c < 2; ++c) put(4 + c, lane, s, w[kW2 + i * 64 + hid(c, g, s)]);
+      // layer 3: [density outputs 4g..4g+3 | SH 4g..4g+3] -> 64; density
+      // output 0 is log-sigma and gets no weight; outputs 1..15 are inputs
+      // 16..30 of W3, SH are inputs 0..15
+      for (int m = 0; m < 4; ++m) {
+        float v;
+        if (s < 4) {
+          const int o2 = 4 * g + s;
+          v = o2 == 0 ? 0.0f : w[kW3 + (16 * m + i) * 31 + 16 + (o2 - 1)];
+        } else {
+          v = w[kW3 + (16 * m + i) * 31 + 4 * g + (s - 4)];
+        }
+        put(6 + m, lane, s, v);
+      }
+      // layer 4: 64 -> 64
+      for (int m = 0; m < 4; ++m)
+        for (int c = 0; c < 2; ++c) put(10 + 2 * m + c, lane, s, w[kW4 + (16 * m + i) * 64 + hid(c, g, s)]);
+      // layer 5: 64 -> 3 (rows 3..15 zero)
+      for (int c = 0; c < 2; ++c) put(18 + c, lane, s, i < 3 ? w[kW5 + i * 64 + hid(c, g, s)] : 0.0f);
+    }
+  }
+}
+
+int scene_upload(SceneDev &s, const nes_scene_desc *desc) {
+  if (!desc || !desc->table || !desc->weights || !desc->occupancy) return NES_EINVAL;
+  const size_t nt = (size_t)kTableEntries * 2;
+  if (!s.table_f32) {
+    NES_HIP_TRY(hipMalloc(&s.table_f32, nt * sizeof(float)));
+    NES_HIP_TRY(hipMalloc(&s.table_f16, nt * sizeof(_Float16)));
+    NES_HIP_TRY(hipMalloc(&s.weights_f32, kWeights * sizeof(float)));
+    NES_HIP_TRY(hipMalloc(&s.wfrag, 20 * 64 * 8 * sizeof(_Float16)));
+    NES_HIP_TRY(hipMalloc(&s.occ, NES_OCC_WORDS * sizeof(uint32_t)));
+  }
+  std::vector<_Float16> t16(nt);
+  for (size_t i = 0; i < nt; ++i) t16[i] = (_Float16)desc->table[i];
+  std::vector<_Float16> frag(20 * 64 * 8);
+  build_mlp_fragments(desc->weights, frag.data());
+  NES_HIP_TRY(hipMemcpy(s.table_f32, desc->table, nt * sizeof(float), hipMemcpyHostToDevice));
+  NES_HIP_TRY(hipMemcpy(s.table_f16, t16.data(), nt * sizeof(_Float16), hipMemcpyHostToDevice));
+  NES_HIP_TRY(hipMemcpy(s.weights_f32, desc->weights, kWeights * sizeof(float), hipMemcpyHostToDevice));
+  NES_HIP_TRY(hipMemcpy(s.wfrag, frag.data(), frag.size() * sizeof(_Float16), hipMemcpyHostToDevice));
+  NES_HIP_TRY(hipMemcpy(s.occ, desc->occupancy, NES_OCC_WORDS * sizeof(uint32_t), hipMemcpyHostToDevice));
+  return NES_OK;
+}
+
+void scene_free(SceneDev &s) {
+  hipFree(s.table_f32);
+  hipFree(s.table_f16);
+  hipFree(s.weights_f32);
+  hipFree(s.wfrag);
+  hipFree(s.occ);
+  s = SceneDev();
+}
+
+void render_work_free(RenderWork &w) {
+  for (DevBuf *b : {&w.views, &w.rays, &w.acc, &w.counts, &w.offsets, &w.samples,
+                    &w.sample_out, &w.out, &w.total})
+    b->release();
+  if (w.total_host) hipHostFree(w.total_host);
+  w.total_host = nullptr;
+  for (hipEvent_t &e : w.ev)
+    if (e) {
+      hipEventDestroy(e);
+      e = nullptr;
+    }
+}
+
+static size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+struct OutLayout {
+  size_t rgb24, depth, rgbf, nsamp, total;
+};
+static OutLayout out_layout(size_t px) {
+  OutLayout l;
+  l.rgb24 = 0;
+  l.depth = align_up(3 * px, 256);
+  l.rgbf = align_up(l.depth + 4 * px, 256);
+  l.nsamp = align_up(l.rgbf + 12 * px, 256);
+  l.total = align_up(l.nsamp + 2 * px, 256);
+  return l;
+}
+
+int render_enqueue(const SceneDev &scene, RenderWork &w, int precision,
+                   const nes_camera *views, int n_views, int steps_per_pass,
+                   hipStream_t stream, nes_render_stats *stats) {
+  if (!views || n_views < 1) return NES_EINVAL;
+  if (precision != NES_PRECISION_FP32 && precision != NES_PRECISION_MIXED) return NES_EINVAL;
+  if (steps_per_pass < 0) return NES_EINVAL;
+  const int spp = steps_per_pass == 0 ? kMaxSteps : std::min(steps_per_pass, kMaxSteps);
+  w.host_views.resize((size_t)n_views);
+  uint64_t slots = 0, px = 0;
+  for (int i = 0; i < n_views; ++i) {
+    const nes_camera &c = views[i];
+    if (c.width < 1 || c.height < 1 || c.width > 16384 || c.height > 16384 ||
+        !(c.fov_y_deg > 0.0f && c.fov_y_deg < 180.0f)) {
+      set_last_error("bad view size or field of view");
+      return NES_EINVAL;
+    }
+    ViewDesc &v = w.host_views[(size_t)i];
+    std::memcpy(v.c2w, c.c2w, sizeof(v.c2w));
+    v.focal = (float)(0.5 * (double)c.height /
+                      std::tan(0.5 * (double)c.fov_y_deg * 3.14159265358979323846 / 180.0));
+    v.width = c.width;
+    v.height = c.height;
+    v.tiles_x = (c.width + 7) / 8;
+    const uint32_t tiles_y = (c.height + 7) / 8;
+    v.slot_base = (uint32_t)slots;
+    v.slot_count = v.tiles_x * tiles_y * 64;
+    v.px_base = (uint32_t)px;
+    v.pad = 0;
+    slots += v.slot_count;
+    px += (uint64_t)c.width * c.height;
+  }
+  if (slots > kMaxSlots) {
+    set_last_error("render batch exceeds 2^25 ray slots");
+    return NES_ERANGE;
+  }
+  w.n_views = (uint32_t)n_views;
+  w.n_slots = (uint32_t)slots;
+  w.out_px = px;
+  const OutLayout L = out_layout(px);
+  NES_TRY(w.views.reserve(sizeof(ViewDesc) * n_views));
+  NES_TRY(w.rays.reserve(sizeof(RayData) * slots));
+  NES_TRY(w.acc.reserve(sizeof(RayAcc) * slots));
+  NES_TRY(w.counts.reserve(sizeof(uint32_t) * slots));
+  NES_TRY(w.offsets.reserve(sizeof(uint32_t) * slots));
+  NES_TRY(w.out.reserve(L.total));
+  NES_TRY(w.total.reserve(64));
+  if (!w.total_host) NES_HIP_TRY(hipHostMalloc((void **)&w.total_host, 64));
+  for (hipEvent_t &e : w.ev)
+    if (!e) NES_HIP_TRY(hipEventCreate(&e));
+
+  NES_HIP_TRY(hipMemcpyAsync(w.views.ptr, w.host_views.data(), sizeof(ViewDesc) * n_views,
+                             hipMemcpyHostToDevice, stream));
+  const unsigned blocks = (unsigned)((slots + 255) / 256);
+  uint32_t *total = w.total.as<uint32_t>();
+  OutPtrs op;
+  op.rgb24 = w.out.as<uint8_t>() + L.rgb24;
+  op.depth = reinterpret_cast<float *>(w.out.as<uint8_t>() + L.depth);
+  op.rgbf = reinterpret_cast<float *>(w.out.as<uint8_t>() + L.rgbf);
+  op.nsamp = reinterpret_cast<uint16_t *>(w.out.as<uint8_t>() + L.nsamp);
+
+  int dev_cu = 256;
+  double ms_march = 0, ms_eval = 0, ms_comp = 0, ms_total = 0;
+  uint64_t n_samples = 0;
+  NES_HIP_TRY(hipEventRecord(w.ev[0], stream));
+  const int passes = (kMaxSteps + spp - 1) / spp;
+  for (int p = 0; p < passes; ++p) {
+    const int k0 = p * spp, k1 = std::min(kMaxSteps, k0 + spp);
+    const bool last = p == passes - 1;
+    zero_u32<<<1, 64, 0, stream>>>(total, 2);
+    NES_HIP_TRY(hipEventRecord(w.ev[1], stream));
+    march_count<<<blocks, 256, 0, stream>>>(w.views.as<ViewDesc>(), w.n_views, w.n_slots, scene.occ,
+                                           w.rays.as<RayData>(), w.acc.as<RayAcc>(),
+                                           w.counts.as<uint32_t>(), total, k0, k1, p == 0);
+    NES_HIP_TRY(hipGetLastError());
+    NES_HIP_TRY(hipMemcpyAsync(w.total_host, total, 4, hipMemcpyDeviceToHost, stream));
+    NES_HIP_TRY(hipStreamSynchronize(stream));
+    const uint32_t n = w.total_host[0];
+    n_samples += n;
+    NES_TRY(w.samples.reserve(sizeof(uint32_t) * ((size_t)n + 1)));
+    NES_TRY(w.sample_out.reserve(sizeof(float4) * ((size_t)n + 1)));
+    march_fill<<<blocks, 256, 0, stream>>>(w.n_slots, scene.occ, w.rays.as<RayData>(),
+                                          w.counts.as<uint32_t>(), w.offsets.as<uint32_t>(),
+                                          w.samples.as<uint32_t>(), total + 1, k0, k1);
+    NES_HIP_TRY(hipGetLastError());
+    NES_HIP_TRY(hipEventRecord(w.ev[2], stream));
+    if (n) {
+      if (precision == NES_PRECISION_FP32) {
+        const unsigned eb = (unsigned)std::min<uint64_t>((n + 255) / 256, (uint64_t)dev_cu * 16);
+        eval_fp32<<<eb, 256, 0, stream>>>(w.samples.as<uint32_t>(), n, w.rays.as<RayData>(),
+                                          scene.table_f32, scene.weights_f32,
+                                          w.sample_out.as<float4>());
+      } else {
+        const uint64_t chunks = (n + 63) / 64;
+        const unsigned eb = (unsigned)std::min<uint64_t>((chunks + kEvalWaves - 1) / kEvalWaves,
+                                                         (uint64_t)dev_cu * 8);
+        eval_mixed<<<eb, 256, 0, stream>>>(w.samples.as<uint32_t>(), n, w.rays.as<RayData>(),
+                                           scene.table_f16,
+                                           reinterpret_cast<const half8 *>(scene.wfrag),
+                                           w.sample_out.as<float4>());
+      }
+      NES_HIP_TRY(hipGetLastError());
+    }
+    NES_HIP_TRY(hipEventRecord(w.ev[3], stream));
+    composite<<<blocks, 256, 0, stream>>>(w.n_slots, w.rays.as<RayData>(), w.acc.as<RayAcc>(),
+                                         w.counts.as<uint32_t>(), w.offsets.as<uint32_t>(),
+                                         w.samples.as<uint32_t>(), w.sample_out.as<float4>(),
+                                         last ? 1 : 0, op);
+    NES_HIP_TRY(hipGetLastError());
+    NES_HIP_TRY(hipEventRecord(w.ev[4], stream));
+    NES_HIP_TRY(hipEventSynchronize(w.ev[4]));
+    float a = 0, b = 0, c = 0;
+    hipEventElapsedTime(&a, w.ev[1], w.ev[2]);
+    hipEventElapsedTime(&b, w.ev[2], w.ev[3]);
+    hipEventElapsedTime(&c, w.ev[3], w.ev[4]);
+    ms_march += a;
+    ms_eval += b;
+    ms_comp += c;
+  }
+  float t = 0;
+  hipEventElapsedTime(&t, w.ev[0], w.ev[4]);
+  ms_total = t;
+  if (stats) {
+    stats->rays = px;
+    stats->samples = n_samples;
+    stats->ms_total = ms_total;
+    stats->ms_eval = ms_eval;
+    stats->ms_march = ms_march;
+    stats->ms_composite = ms_comp;
+  }
+  return NES_OK;
+}
+
+int render_fetch(RenderWork &w, int view, const nes_view_out *out, hipStream_t stream) {
+  if (!out || view < 0 || (uint32_t)view >= w.n_views) return NES_EINVAL;
+  const ViewDesc &v = w.host_views[(size_t)view];
+  const size_t px = (size_t)v.width * v.height, base = v.px_base;
+  const OutLayout L = out_layout(w.out_px);
+  const uint8_t *o = w.out.as<uint8_t>();
+  if (out->rgb24)
+    NES_HIP_TRY(hipMemcpyAsync(out->rgb24, o + L.rgb24 + 3 * base, 3 * px, hipMemcpyDeviceToHost, stream));
+  if (out->depth)
+    NES_HIP_TRY(hipMemcpyAsync(out->depth, o + L.depth + 4 * base, 4 * px, hipMemcpyDeviceToHost, stream));
+  if (out->rgb_f32)
+    NES_HIP_TRY(hipMemcpyAsync(out->rgb_f32, o + L.rgbf + 12 * base, 12 * px, hipMemcpyDeviceToHost, stream));
+  if (out->n_samples)
+    NES_HIP_TRY(hipMemcpyAsync(out->n_samples, o + L.nsamp + 2 * base, 2 * px, hipMemcpyDeviceToHost, stream));
+  NES_HIP_TRY(hipStreamSynchronize(stream));
+  return NES_OK;
+}
+
+}  // namespace nes
+
+using namespace nes;
+
+extern "C" {
+
+int nes_ctx_create(const nes_scene_desc *scene, uint64_t hbm_budget_bytes, nes_ctx **out) {
+  if (!scene || !out) return NES_EINVAL;
+  NES_TRY(ensure_device());
+  nes_ctx *c = new nes_ctx();
+  c->budget = hbm_budget_bytes ? hbm_budget_bytes : (16ull << 30);
+  int rc = scene_upload(c->scene, scene);
+  if (rc == NES_OK && hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) rc = NES_EIO;
+  if (rc != NES_OK) {
+    scene_free(c->scene);
+    delete c;
+    return rc;
+  }
+  *out = c;
+  return NES_OK;
+}
+
+int nes_ctx_destroy(nes_ctx *ctx) {
+  if (!ctx) return NES_EINVAL;
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    render_work_free(ctx->work);
+    scene_free(ctx->scene);
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
+  }
+  delete ctx;
+  return NES_OK;
+}
+
+int nes_ctx_update_scene(nes_ctx *ctx, const nes_scene_desc *scene) {
+  if (!ctx) return NES_EINVAL;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  NES_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return scene_upload(ctx->scene, scene);
+}
+
+int nes_render_device(nes_ctx *ctx, int precision, const nes_camera *views, int n_views,
+                      int steps_per_pass, nes_render_stats *stats) {
+  if (!ctx) return NES_EINVAL;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  return render_enqueue(ctx->scene, ctx->work, precision, views, n_views, steps_per_pass,
+                        ctx->stream, stats);
+}
+
+int nes_render_fetch(nes_ctx *ctx, int view, const nes_view_out *out) {
+  if (!ctx) return NES_EINVAL;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  return render_fetch(ctx->work, view, out, ctx->stream);
+}
+
+int nes_render(nes_ctx *ctx, int precision, const nes_camera *views, int n_views,
+               const nes_view_out *outs, int steps_per_pass, nes_render_stats *stats) {
+  if (!ctx || !outs) return NES_EINVAL;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  NES_TRY(render_enqueue(ctx->scene, ctx->work, precision, views, n_views, steps_per_pass,
+                         ctx->stream, stats));
+  for (int i = 0; i < n_views; ++i) NES_TRY(render_fetch(ctx->work, i, &outs[i], ctx->stream));
+  return NES_OK;
+}
+
+}  // extern "C"

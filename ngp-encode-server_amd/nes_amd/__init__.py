@@ -1,0 +1,312 @@
+"""Host-side binding of libnes_render.so (ctypes; no GPU framework needed).
+
+The shipped compute path is the HIP library; this module only marshals
+arguments.  It mirrors the reference's operator boundary for the render +
+frame-prep path:
+
+* ``Context.render`` replaces one FrameRequest -> RenderedFrame round trip of
+  ``socket_client_thread`` (src/server.cpp:140-194);
+* ``frameprep`` replaces ``RenderedFrame::convert_frame`` /
+  ``SwsContextManager`` (include/base/video/rendered_frame.h:24-33,
+  src/base/video/type_managers.cc:143-155).
+
+Errors surface as :class:`NesError` carrying the library's negative errno code;
+there is no fallback path -- a missing or broken library raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(HERE), "lib", "libnes_render.so")
+
+NES_OK = 0
+NES_EAGAIN = -11
+PRECISION_FP32 = 0
+PRECISION_MIXED = 1
+SWS_X86 = 0
+SWS_C = 1
+TABLE_ENTRIES = 6098925
+MLP_WEIGHTS = 9344
+OCC_WORDS = 65536
+
+
+class NesError(RuntimeError):
+    def __init__(self, code, where, detail=""):
+        self.code = code
+        super().__init__(f"{where} failed: {code} ({detail})")
+
+
+class Camera(C.Structure):
+    """nes_camera: nesproto::Camera + FrameRequest.index + vertical FOV."""
+    _fields_ = [("index", C.c_uint64), ("is_left", C.c_uint32), ("width", C.c_uint32),
+                ("height", C.c_uint32), ("c2w", C.c_float * 12), ("fov_y_deg", C.c_float)]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [("table", C.POINTER(C.c_float)), ("weights", C.POINTER(C.c_float)),
+                ("occupancy", C.POINTER(C.c_uint32))]
+
+
+class ViewOut(C.Structure):
+    _fields_ = [("rgb24", C.POINTER(C.c_uint8)), ("depth", C.POINTER(C.c_float)),
+                ("rgb_f32", C.POINTER(C.c_float)), ("n_samples", C.POINTER(C.c_uint16))]
+
+
+class RenderStats(C.Structure):
+    _fields_ = [("rays", C.c_uint64), ("samples", C.c_uint64), ("ms_total", C.c_double),
+                ("ms_eval", C.c_double), ("ms_march", C.c_double), ("ms_composite", C.c_double)]
+
+
+# C-ABI symbols declared in include/nes_render.h (checked by the CPU tests).
+_SIGS = {}
+
+
+def _sig(name, res, *args):
+    _SIGS[name] = (res, list(args))
+
+
+_P = C.POINTER
+_sig("nes_strerror", C.c_char_p, C.c_int)
+_sig("nes_last_error", C.c_char_p)
+_sig("nes_scene_synthesize", C.c_int, C.c_uint64, C.c_int, _P(C.c_float), _P(C.c_float),
+     _P(C.c_uint32))
+_sig("nes_orbit_camera", C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, _P(C.c_float))
+_sig("nes_ctx_create", C.c_int, _P(SceneDesc), C.c_uint64, _P(C.c_void_p))
+_sig("nes_ctx_destroy", C.c_int, C.c_void_p)
+_sig("nes_ctx_update_scene", C.c_int, C.c_void_p, _P(SceneDesc))
+_sig("nes_render", C.c_int, C.c_void_p, C.c_int, _P(Camera), C.c_int, _P(ViewOut), C.c_int,
+     _P(RenderStats))
+_sig("nes_render_device", C.c_int, C.c_void_p, C.c_int, _P(Camera), C.c_int, C.c_int,
+     _P(RenderStats))
+_sig("nes_render_fetch", C.c_int, C.c_void_p, C.c_int, _P(ViewOut))
+_sig("nes_frameprep", C.c_int, _P(C.c_uint8), _P(C.c_float), _P(C.c_uint8), _P(C.c_float),
+     C.c_int, C.c_int, C.c_int, _P(C.c_uint8), C.c_int, _P(C.c_uint8), C.c_int, _P(C.c_uint8),
+     C.c_int, C.c_int)
+_sig("nes_frameprep_device", C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+     C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int,
+     _P(C.c_float))
+_sig("nes_gray8_to_yuv420p", C.c_int, _P(C.c_uint8), C.c_int, C.c_int, C.c_int, _P(C.c_uint8),
+     C.c_int, _P(C.c_uint8), C.c_int, _P(C.c_uint8), C.c_int, C.c_int)
+_sig("nes_sws_vchroma_filter", C.c_int, C.c_int, _P(C.c_int16), _P(C.c_int32), C.c_int,
+     _P(C.c_int))
+_sig("nes_device_alloc", C.c_int, C.c_size_t, _P(C.c_void_p))
+_sig("nes_device_free", C.c_int, C.c_void_p)
+_sig("nes_memcpy_h2d", C.c_int, C.c_void_p, C.c_void_p, C.c_size_t)
+_sig("nes_memcpy_d2h", C.c_int, C.c_void_p, C.c_void_p, C.c_size_t)
+_sig("nes_device_info", C.c_int, C.c_char_p, C.c_size_t, _P(C.c_int), _P(C.c_uint64))
+
+_lib = None
+
+
+def lib():
+    """Load libnes_render.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError(f"{LIB_PATH} not built; run ngp-encode-server_amd/build.py")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            if hasattr(L, name):
+                fn = getattr(L, name)
+                fn.restype = res
+                fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc, where):
+    if rc != NES_OK:
+        detail = lib().nes_last_error().decode(errors="replace")
+        raise NesError(rc, where, f"{lib().nes_strerror(rc).decode()}; {detail}")
+
+
+def _ptr(a, t):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+# ----------------------------------------------------------------- scene
+
+def scene_synthesize(seed, table_init=0):
+    table = np.empty(TABLE_ENTRIES * 2, np.float32)
+    weights = np.empty(MLP_WEIGHTS, np.float32)
+    occ = np.empty(OCC_WORDS, np.uint32)
+    check(lib().nes_scene_synthesize(seed, table_init, _ptr(table, C.c_float),
+                                     _ptr(weights, C.c_float), _ptr(occ, C.c_uint32)),
+          "nes_scene_synthesize")
+    return table, weights, occ
+
+
+def orbit_camera(azimuth, elevation, radius=2.0, eye_offset=0.0):
+    m = np.empty(12, np.float32)
+    check(lib().nes_orbit_camera(azimuth, elevation, radius, eye_offset, _ptr(m, C.c_float)),
+          "nes_orbit_camera")
+    return m
+
+
+def camera(width, height, c2w, fov=60.0, index=0, is_left=1):
+    cam = Camera()
+    cam.index, cam.is_left, cam.width, cam.height = index, is_left, width, height
+    cam.c2w[:] = [float(x) for x in c2w]
+    cam.fov_y_deg = fov
+    return cam
+
+
+class Context:
+    """A device-resident scene (nes_ctx)."""
+
+    def __init__(self, table, weights, occ, hbm_budget=0):
+        self._keep = (np.ascontiguousarray(table, np.float32),
+                      np.ascontiguousarray(weights, np.float32),
+                      np.ascontiguousarray(occ, np.uint32))
+        desc = SceneDesc(_ptr(self._keep[0], C.c_float), _ptr(self._keep[1], C.c_float),
+                         _ptr(self._keep[2], C.c_uint32))
+        h = C.c_void_p()
+        check(lib().nes_ctx_create(C.byref(desc), hbm_budget, C.byref(h)), "nes_ctx_create")
+        self.handle = h
+
+    def update_scene(self, table, weights, occ):
+        keep = (np.ascontiguousarray(table, np.float32), np.ascontiguousarray(weights, np.float32),
+                np.ascontiguousarray(occ, np.uint32))
+        desc = SceneDesc(_ptr(keep[0], C.c_float), _ptr(keep[1], C.c_float),
+                         _ptr(keep[2], C.c_uint32))
+        check(lib().nes_ctx_update_scene(self.handle, C.byref(desc)), "nes_ctx_update_scene")
+        self._keep = keep
+
+    def render(self, cams, precision=PRECISION_MIXED, steps_per_pass=0):
+        """Render views; returns ([dict per view], RenderStats)."""
+        if isinstance(cams, Camera):
+            cams = [cams]
+        n = len(cams)
+        arr = (Camera * n)(*cams)
+        outs = (ViewOut * n)()
+        results = []
+        for i, c in enumerate(cams):
+            r = {"rgb24": np.empty((c.height, c.width, 3), np.uint8),
+                 "depth": np.empty((c.height, c.width), np.float32),
+                 "rgb": np.empty((c.height, c.width, 3), np.float32),
+                 "n_samples": np.empty((c.height, c.width), np.uint16)}
+            outs[i] = ViewOut(_ptr(r["rgb24"], C.c_uint8), _ptr(r["depth"], C.c_float),
+                              _ptr(r["rgb"], C.c_float), _ptr(r["n_samples"], C.c_uint16))
+            results.append(r)
+        st = RenderStats()
+        check(lib().nes_render(self.handle, precision, arr, n, outs, steps_per_pass, C.byref(st)),
+              "nes_render")
+        return results, st
+
+    def render_device(self, cams, precision=PRECISION_MIXED, steps_per_pass=0):
+        if isinstance(cams, Camera):
+            cams = [cams]
+        arr = (Camera * len(cams))(*cams)
+        st = RenderStats()
+        check(lib().nes_render_device(self.handle, precision, arr, len(cams), steps_per_pass,
+                                      C.byref(st)), "nes_render_device")
+        return st
+
+    def close(self):
+        if self.handle:
+            check(lib().nes_ctx_destroy(self.handle), "nes_ctx_destroy")
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------ frame prep
+
+def frameprep(rgb_a, depth_a=None, rgb_b=None, depth_b=None, mode=SWS_X86):
+    """Nearer-depth composite + RGB24 -> YUV420P (host arrays); returns (y, u, v)."""
+    h, w, _ = rgb_a.shape
+    rgb_a = np.ascontiguousarray(rgb_a, np.uint8)
+    if depth_a is not None:
+        depth_a = np.ascontiguousarray(depth_a, np.float32)
+    if rgb_b is not None:
+        rgb_b = np.ascontiguousarray(rgb_b, np.uint8)
+        depth_b = np.ascontiguousarray(depth_b, np.float32)
+    y = np.empty((h, w), np.uint8)
+    u = np.empty((h // 2, w // 2), np.uint8)
+    v = np.empty((h // 2, w // 2), np.uint8)
+    check(lib().nes_frameprep(_ptr(rgb_a, C.c_uint8), _ptr(depth_a, C.c_float),
+                              _ptr(rgb_b, C.c_uint8), _ptr(depth_b, C.c_float), w, h, 3 * w,
+                              _ptr(y, C.c_uint8), w, _ptr(u, C.c_uint8), w // 2,
+                              _ptr(v, C.c_uint8), w // 2, mode), "nes_frameprep")
+    return y, u, v
+
+
+def gray8_to_yuv420p(g, mode=SWS_X86):
+    h, w = g.shape
+    g = np.ascontiguousarray(g, np.uint8)
+    y = np.empty((h, w), np.uint8)
+    u = np.empty((h // 2, w // 2), np.uint8)
+    v = np.empty((h // 2, w // 2), np.uint8)
+    check(lib().nes_gray8_to_yuv420p(_ptr(g, C.c_uint8), w, h, w, _ptr(y, C.c_uint8), w,
+                                     _ptr(u, C.c_uint8), w // 2, _ptr(v, C.c_uint8), w // 2,
+                                     mode), "nes_gray8_to_yuv420p")
+    return y, u, v
+
+
+def vchroma_filter(h, max_taps=16):
+    ch = h // 2
+    coef = np.zeros(ch * max_taps, np.int16)
+    pos = np.zeros(ch, np.int32)
+    taps = C.c_int()
+    check(lib().nes_sws_vchroma_filter(h, _ptr(coef, C.c_int16), _ptr(pos, C.c_int32), max_taps,
+                                       C.byref(taps)), "nes_sws_vchroma_filter")
+    return coef[: ch * taps.value].reshape(ch, taps.value), pos
+
+
+class DeviceBuffer:
+    """Raw device allocation through the library (no other GPU runtime)."""
+
+    def __init__(self, nbytes):
+        p = C.c_void_p()
+        check(lib().nes_device_alloc(nbytes, C.byref(p)), "nes_device_alloc")
+        self.ptr, self.nbytes = p, nbytes
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        check(lib().nes_memcpy_h2d(self.ptr, arr.ctypes.data_as(C.c_void_p), arr.nbytes),
+              "nes_memcpy_h2d")
+
+    def download(self, arr):
+        assert arr.nbytes <= self.nbytes and arr.flags["C_CONTIGUOUS"]
+        check(lib().nes_memcpy_d2h(arr.ctypes.data_as(C.c_void_p), self.ptr, arr.nbytes),
+              "nes_memcpy_d2h")
+        return arr
+
+    def free(self):
+        if self.ptr:
+            check(lib().nes_device_free(self.ptr), "nes_device_free")
+            self.ptr = None
+
+
+def frameprep_device(rgb_a, depth_a, rgb_b, depth_b, w, h, n_frames, y, u, v, mode=SWS_X86):
+    """Device-pointer frame prep over n_frames packed frames; returns kernel ms."""
+    ms = C.c_float()
+    vp = lambda b: b.ptr if b is not None else None
+    check(lib().nes_frameprep_device(vp(rgb_a), vp(depth_a), vp(rgb_b), vp(depth_b), w, h,
+                                     n_frames, vp(y), w, vp(u), w // 2, vp(v), w // 2, mode,
+                                     C.byref(ms)), "nes_frameprep_device")
+    return ms.value
+
+
+def device_info():
+    name = C.create_string_buffer(64)
+    cus = C.c_int()
+    hbm = C.c_uint64()
+    check(lib().nes_device_info(name, 64, C.byref(cus), C.byref(hbm)), "nes_device_info")
+    return name.value.decode(), cus.value, hbm.value

@@ -1,0 +1,148 @@
+"""Render kernels against the FP64 oracle (the same FP32 sample geometry, so
+sample sets must agree exactly; colour and depth within stated tolerances)."""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+FP32, MIXED = 0, 1
+# FP32 path: per-pixel colour within 2e-5 and depth within 1e-4 of FP64.
+FP32_RGB_TOL, FP32_DEPTH_TOL = 2e-5, 1e-4
+# Mixed path (FP16 table + FP16 MFMA MLPs): >= 45 dB PSNR on the float image.
+MIXED_PSNR_DB = 45.0
+
+
+def psnr(a, b):
+    mse = float(np.mean((np.asarray(a, np.float64) - np.asarray(b, np.float64)) ** 2))
+    return float("inf") if mse == 0 else 10 * math.log10(1.0 / mse)
+
+
+def cams(nes, oracle, w, h, az=30.0, el=20.0, eye=0.0):
+    c2w = nes.orbit_camera(az, el, 2.0, eye)
+    return nes.camera(w, h, c2w), oracle.camera(w, h, c2w)
+
+
+def oracle_view(oracle, scene, ocam, pix):
+    t, w, o = scene
+    return oracle.render_pixels(t, w, o, ocam, pix, threads=16)
+
+
+@pytest.mark.parametrize("prec", [FP32, MIXED])
+def test_full_image_160x120(nes, oracle, gpu_ctx, scene1, prec):
+    cam, ocam = cams(nes, oracle, 160, 120)
+    (out,), st = gpu_ctx.render(cam, precision=prec)
+    pix = np.arange(160 * 120, dtype=np.uint32)
+    rgb, depth, ns = oracle_view(oracle, scene1, ocam, pix)
+    assert np.array_equal(out["n_samples"].reshape(-1), ns), "sample sets differ"
+    assert st.samples == ns.sum()
+    got = out["rgb"].reshape(-1, 3)
+    if prec == FP32:
+        assert np.abs(got - rgb).max() < FP32_RGB_TOL
+        assert np.abs(out["depth"].reshape(-1) - depth).max() < FP32_DEPTH_TOL
+    else:
+        assert psnr(got, rgb) >= MIXED_PSNR_DB
+        hit = ns > 0
+        assert psnr(got[hit], rgb[hit]) >= MIXED_PSNR_DB - 6
+    q = np.array([[oracle.lib().orc_quantize(c) for c in px] for px in rgb[::97]])
+    assert np.abs(out["rgb24"].reshape(-1, 3)[::97].astype(int) - q).max() <= 1
+
+
+@pytest.mark.parametrize("prec", [FP32, MIXED])
+def test_config1_640x480_subset(nes, oracle, gpu_ctx, scene1, prec):
+    cam, ocam = cams(nes, oracle, 640, 480)
+    (out,), st = gpu_ctx.render(cam, precision=prec)
+    pix = np.arange(0, 640 * 480, 7, dtype=np.uint32)
+    rgb, depth, ns = oracle_view(oracle, scene1, ocam, pix)
+    assert np.array_equal(out["n_samples"].reshape(-1)[pix], ns)
+    got = out["rgb"].reshape(-1, 3)[pix]
+    p = psnr(got, rgb)
+    print(f"precision {prec}: PSNR {p:.1f} dB vs FP64 oracle on {len(pix)} pixels, "
+          f"max depth err {np.abs(out['depth'].reshape(-1)[pix] - depth).max():.2e}")
+    assert p >= (90.0 if prec == FP32 else MIXED_PSNR_DB)
+
+
+@pytest.mark.parametrize("prec", [FP32, MIXED])
+def test_passes_and_batching_are_invariant(nes, oracle, gpu_ctx, prec):
+    """Marching in passes of 16 steps, or rendering a view inside a stereo
+    batch, gives bit-identical results."""
+    left, _ = cams(nes, oracle, 320, 240, eye=-0.032)
+    right, _ = cams(nes, oracle, 320, 240, eye=0.032)
+    (a,), _ = gpu_ctx.render(left, precision=prec)
+    (b,), _ = gpu_ctx.render(left, precision=prec, steps_per_pass=16)
+    (c, d), _ = gpu_ctx.render([left, right], precision=prec)
+    (e,), _ = gpu_ctx.render(right, precision=prec)
+    for k in ("rgb", "depth", "n_samples", "rgb24"):
+        assert np.array_equal(a[k], b[k]) and np.array_equal(a[k], c[k]), k
+        assert np.array_equal(d[k], e[k]), k
+
+
+def test_mixed_resolution_batch(nes, oracle, gpu_ctx):
+    views = [cams(nes, oracle, w, h, az=az)[0] for (w, h, az) in
+             [(64, 48, 10.0), (200, 130, 40.0), (96, 96, 80.0)]]
+    outs, _ = gpu_ctx.render(views, precision=MIXED)
+    for v, o in zip(views, outs):
+        (solo,), _ = gpu_ctx.render(v, precision=MIXED)
+        assert np.array_equal(solo["rgb"], o["rgb"]) and np.array_equal(solo["depth"], o["depth"])
+
+
+def _const_scene(sigma):
+    table = np.full(6098925 * 2, 1.0, np.float32)
+    w = np.zeros(9344, np.float32)
+    w[0:2048] = 1.0 / 32.0
+    w[2048:2048 + 64] = math.log(sigma) / 64.0
+    occ = np.full(65536, 0xFFFFFFFF, np.uint32)
+    return table, w, occ
+
+
+@pytest.mark.parametrize("prec", [FP32, MIXED])
+def test_dense_scene_early_termination(nes, oracle, prec):
+    """Opaque constant density: every ray stops at T < 1e-4 after the same
+    number of samples as the oracle; passes of 8 steps give identical output."""
+    scene = _const_scene(1500.0)
+    cam, ocam = cams(nes, oracle, 96, 72)
+    with nes.Context(*scene) as ctx:
+        (a,), st = ctx.render(cam, precision=prec)
+        (b,), _ = ctx.render(cam, precision=prec, steps_per_pass=8)
+    pix = np.arange(96 * 72, dtype=np.uint32)
+    rgb, depth, ns = oracle.render_pixels(*scene, ocam, pix, threads=16)
+    hit = ns > 0
+    assert hit.mean() > 0.3
+    assert np.abs(a["n_samples"].reshape(-1).astype(int) - ns).max() <= (0 if prec == FP32 else 1)
+    assert np.abs(a["rgb"].reshape(-1, 3) - rgb).max() < (1e-5 if prec == FP32 else 2e-3)
+    for k in ("rgb", "depth", "n_samples"):
+        assert np.array_equal(a[k], b[k])
+
+
+def test_uniform_slab_closed_form_gpu(nes, oracle):
+    """Zero network: sigma = 1, colour 0.5; C = 0.5 (1 - exp(-n delta))."""
+    table = np.zeros(6098925 * 2, np.float32)
+    w = np.zeros(9344, np.float32)
+    occ = np.full(65536, 0xFFFFFFFF, np.uint32)
+    cam, _ = cams(nes, oracle, 64, 64)
+    with nes.Context(table, w, occ) as ctx:
+        for prec in (FP32, MIXED):
+            (o,), _ = ctx.render(cam, precision=prec)
+            n = o["n_samples"].astype(np.float64)
+            want = 0.5 * (1 - np.exp(-n * float(oracle.DELTA)))
+            assert np.abs(o["rgb"][..., 0] - want).max() < 2e-6
+
+
+def test_config2_stereo_1080p_subset(nes, oracle, gpu_ctx, scene1):
+    """Headline configuration: 1920x1080 per eye, 64 mm IPD; pixels on a
+    sparse lattice against the oracle, full frames for determinism."""
+    L, oL = cams(nes, oracle, 1920, 1080, eye=-0.032)
+    R, oR = cams(nes, oracle, 1920, 1080, eye=0.032)
+    outs, st = gpu_ctx.render([L, R], precision=MIXED)
+    outs2, st2 = gpu_ctx.render([L, R], precision=MIXED)
+    assert st.samples == st2.samples and st.rays == 2 * 1920 * 1080
+    pix = np.arange(0, 1920 * 1080, 61, dtype=np.uint32)
+    for o, o2, oc in zip(outs, outs2, (oL, oR)):
+        assert np.array_equal(o["rgb"], o2["rgb"])
+        rgb, depth, ns = oracle_view(oracle, scene1, oc, pix)
+        assert np.array_equal(o["n_samples"].reshape(-1)[pix], ns)
+        p = psnr(o["rgb"].reshape(-1, 3)[pix], rgb)
+        print(f"stereo eye PSNR {p:.1f} dB, max depth err "
+              f"{np.abs(o['depth'].reshape(-1)[pix] - depth).max():.2e}")
+        assert p >= MIXED_PSNR_DB

@@ -1,0 +1,157 @@
+"""Pins of the libswscale RGB24->YUV420P restatement (and the nearer-depth
+composite) against closed forms.  libswscale itself is absent from the
+reference tree, this container and the GPU image, so its bit output is not
+available as a fixture: the restatement is pinned by the BT.601 formulas, the
+bicubic kernel, and invariants that any slip in a coefficient, shift, tap or
+border rule breaks."""
+import math
+
+import numpy as np
+import pytest
+
+
+def bt601(rgb):
+    r, g, b = (rgb[..., i].astype(np.float64) for i in range(3))
+    y = 16 + 219 / 255 * (0.299 * r + 0.587 * g + 0.114 * b)
+    u = 128 + 224 / 255 * (-0.169 * r - 0.331 * g + 0.5 * b)
+    v = 128 + 224 / 255 * (0.5 * r - 0.419 * g - 0.081 * b)
+    return y, u, v
+
+
+def keys_cubic(x, c=0.6):
+    """libswscale's bicubic with B = 0, C = 0.6 (Keys a = -0.6)."""
+    x = abs(x)
+    if x < 1:
+        return ((12 - 6 * c) * x ** 3 + (-18 + 6 * c) * x ** 2 + 6) / 6
+    if x < 2:
+        return (-6 * c * x ** 3 + 30 * c * x ** 2 - 48 * c * x + 24 * c) / 6
+    return 0.0
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_luma_matches_bt601(oracle, mode):
+    rng = np.random.default_rng(0)
+    img = rng.integers(0, 256, (64, 96, 3), dtype=np.uint8)
+    y, _, _ = oracle.rgb24_to_yuv420p(img, mode)
+    yf, _, _ = bt601(img)
+    # 15-bit coefficients and two rounding stages: within half a level + 1/64
+    assert np.abs(y - yf).max() <= 0.5 + 1 / 64 + 1e-9
+    assert y.min() >= 16 and y.max() <= 235
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_constant_images_give_constant_planes(oracle, mode):
+    rng = np.random.default_rng(1)
+    for rgb in [(0, 0, 0), (255, 255, 255), (255, 0, 0), (0, 255, 0), (0, 0, 255),
+                tuple(int(x) for x in rng.integers(0, 256, 3))]:
+        img = np.empty((40, 24, 3), np.uint8)
+        img[:] = rgb
+        y, u, v = oracle.rgb24_to_yuv420p(img, mode)
+        assert (y == y.flat[0]).all(), rgb
+        for p in (u, v):
+            if mode == 1:
+                assert (p == p.flat[0]).all(), rgb
+            else:
+                # pmulhw truncation depends on the per-row taps (border rows
+                # and the C-path last row), so rows may differ by one level
+                assert int(p.max()) - int(p.min()) <= 1, rgb
+                assert (p[2:-2] == p[2, 0]).all(), rgb
+        yf, uf, vf = bt601(np.array(rgb, np.uint8))
+        assert abs(int(y[0, 0]) - float(yf)) <= 0.52
+        # chroma: pmulhw truncation in the x86 path can lower it by one more
+        assert abs(int(u[0, 0]) - float(uf)) <= 1.05 and abs(int(v[0, 0]) - float(vf)) <= 1.05
+    black = np.zeros((8, 8, 3), np.uint8)
+    assert (oracle.rgb24_to_yuv420p(black, mode)[0] == 16).all()
+    assert (oracle.rgb24_to_yuv420p(black + 255, mode)[0] == 235).all()
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_gray_rgb_has_neutral_chroma(oracle, mode):
+    """R = G = B: the chroma coefficients sum to -1/2^15 and the rounding term
+    absorbs it, so U = V = 128 exactly, whatever the picture."""
+    rng = np.random.default_rng(2)
+    g = rng.integers(0, 256, (48, 64), dtype=np.uint8)
+    img = np.repeat(g[..., None], 3, axis=2)
+    _, u, v = oracle.rgb24_to_yuv420p(img, mode)
+    assert (u == 128).all() and (v == 128).all()
+
+
+def test_vertical_filter_is_the_bicubic_kernel(oracle):
+    coef, pos = oracle.vchroma_filter(720)
+    assert coef.shape == (360, 8)
+    assert (coef.sum(axis=1) == 4096).all()
+    # interior: taps at source rows 2k-3 .. 2k+4, distances (2j-7)/2 source
+    # pixels from the chroma centre at 2k+0.5, kernel stretched by 2
+    want = np.array([keys_cubic((2 * j - 7) / 4) / 2 * 4096 for j in range(8)])
+    for k in range(2, 358):
+        assert pos[k] == 2 * k - 3
+        assert np.abs(coef[k] - want).max() <= 1.0
+    # borders fold the outside taps onto the edge row
+    assert pos[0] == 0 and pos[1] == 0 and pos[358] == 712 and pos[359] == 712
+    assert abs(int(coef[0][0]) - want[:4].sum()) <= 2 and np.abs(coef[0][1:5] - want[4:]).max() <= 1
+    assert abs(int(coef[359][7]) - want[4:].sum()) <= 2
+
+
+def test_filter_shapes_small_heights(oracle):
+    for h in range(2, 40, 2):
+        coef, pos = oracle.vchroma_filter(h)
+        assert coef.shape[0] == h // 2
+        assert (coef.sum(axis=1) >= 4090).all() and (coef.sum(axis=1) <= 4102).all()
+        assert (pos >= 0).all() and (pos + coef.shape[1] <= h).all()
+
+
+def test_x86_and_c_paths_differ_by_at_most_one(oracle):
+    rng = np.random.default_rng(3)
+    img = rng.integers(0, 256, (96, 128, 3), dtype=np.uint8)
+    y0, u0, v0 = oracle.rgb24_to_yuv420p(img, 0)
+    y1, u1, v1 = oracle.rgb24_to_yuv420p(img, 1)
+    assert (y0 == y1).all()
+    for a, b in ((u0, u1), (v0, v1)):
+        d = a.astype(int) - b.astype(int)
+        assert np.abs(d).max() <= 1 and (d != 0).any()
+        # the last chroma row is produced by the C output function in both
+        assert (d[-1] == 0).all()
+
+
+def test_chroma_matches_float_reference(oracle):
+    """Chroma of a smooth picture equals the float BT.601 chroma of the
+    pair-averaged pixels, filtered by the bicubic kernel, within one level."""
+    yy, xx = np.mgrid[0:64, 0:64]
+    img = np.stack([(xx * 4) % 256, (yy * 3) % 256, (xx + yy) % 256], -1).astype(np.uint8)
+    _, u, v = oracle.rgb24_to_yuv420p(img, 1)
+    _, uf, vf = bt601(img)
+    uf = 0.5 * (uf[:, 0::2] + uf[:, 1::2])
+    coef, pos = oracle.vchroma_filter(64)
+    ur = np.array([[sum(coef[k][j] / 4096 * uf[pos[k] + j, i] for j in range(8))
+                    for i in range(32)] for k in range(32)])
+    assert np.abs(u - ur).max() <= 1.0
+
+
+def test_composite_brute_force(oracle):
+    a = np.arange(3 * 4 * 3, dtype=np.uint8).reshape(3, 4, 3)
+    b = (255 - a).astype(np.uint8)
+    nan, inf = np.float32("nan"), np.float32("inf")
+    da = np.array([[1, 2, 3, nan], [1, nan, -0.0, inf], [5, 5, 0.1, 7]], np.float32)
+    db = np.array([[2, 2, 1, 1], [nan, nan, 0.0, inf], [4, 6, 0.1, inf]], np.float32)
+    out, dout = oracle.composite(a, da, b, db)
+    for r in range(3):
+        for c in range(4):
+            ka = np.inf if np.isnan(da[r, c]) else da[r, c]
+            kb = np.inf if np.isnan(db[r, c]) else db[r, c]
+            want = a[r, c] if ka <= kb else b[r, c]
+            assert (out[r, c] == want).all(), (r, c)
+
+
+def test_gray8_depth_stream(oracle):
+    g = np.tile(np.arange(256, dtype=np.uint8), (4, 1))[:, :256]
+    y, u, v = oracle.gray8_to_yuv420p(g)
+    assert y[0, 0] == 16 and y[0, 255] == 235
+    assert (np.diff(y[0].astype(int)) >= 0).all()
+    assert np.abs(y[0] - (16 + 219 / 255 * np.arange(256))).max() <= 0.51
+    assert (u == 128).all() and (v == 128).all()
+
+
+def test_odd_sizes_rejected(oracle):
+    img = np.zeros((5, 4, 3), np.uint8)
+    with pytest.raises(ValueError):
+        oracle.rgb24_to_yuv420p(img)
