@@ -1,0 +1,180 @@
+"""Benchmark: mixed-precision stereo render throughput on one MI355X.
+
+One step = one render of a stereo pair, 1920x1080 per eye with 64 mm IPD
+(4,147,200 rays), seeded scene (16-level hash grid, 128^3 occupancy blob,
+Xavier MLPs), through the C-ABI (nes_render_device: outputs stay in HBM).
+Prints one JSON line.  --gpus above 1 is refused: this build targets one GPU.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "ngp-encode-server_amd"))
+
+METRIC = "Mrays/s mixed-prec vs FP32 at ≥45 dB PSNR; p99 frame latency, 8 clients"
+FLOP_PER_SAMPLE = 18688      # density 32-64-16 + colour 31-64-64-3 MLPs, 2 FLOP/MAC
+PEAK_FP16_TFLOPS = 2500.0    # MI355X dense FP16 MFMA
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def stereo_views(N, w=1920, h=1080):
+    return [N.camera(w, h, N.orbit_camera(30.0, 20.0, 2.0, e), fov=60.0, is_left=int(e < 0))
+            for e in (-0.032, 0.032)]
+
+
+def time_render(N, ctx, views, precision, steps, warmup):
+    for _ in range(warmup):
+        ctx.render_device(views, precision)
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        stats.append(ctx.render_device(views, precision))
+    wall = time.perf_counter() - t0
+    return wall, stats
+
+
+def cpu_baseline(N, table, weights, occ, ctx, views, budget_s):
+    """FP64 oracle on a bounded, evenly strided subset of the same rays."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    threads = min(16, os.cpu_count() or 1)
+    outs, _ = ctx.render(views, precision=N.PRECISION_MIXED)
+    w, h = views[0].width, views[0].height
+    stride = 9
+    total_rays, total_t = 0, 0.0
+    ref, got = [], []
+    for v, o in zip(views, outs):
+        ocam = O.camera(w, h, list(v.c2w), fov=v.fov_y_deg)
+        pix = np.arange(0, w * h, stride, dtype=np.uint32)
+        # time a slice first to bound the run to about budget_s seconds
+        probe = pix[:4096]
+        t0 = time.perf_counter()
+        O.render_pixels(table, weights, occ, ocam, probe, threads=threads)
+        dt = time.perf_counter() - t0
+        keep = int(min(len(pix), max(4096, budget_s / 2 / max(dt, 1e-9) * 4096)))
+        pix = pix[:: max(1, len(pix) // keep)]
+        t0 = time.perf_counter()
+        rgb, depth, ns = O.render_pixels(table, weights, occ, ocam, pix, threads=threads)
+        total_t += time.perf_counter() - t0
+        total_rays += len(pix)
+        ref.append(rgb)
+        got.append(o["rgb"].reshape(-1, 3)[pix])
+    ref, got = np.concatenate(ref), np.concatenate(got)
+    mse = float(np.mean((ref - got) ** 2))
+    psnr = 10 * math.log10(1.0 / mse) if mse > 0 else float("inf")
+    base = {"value": total_rays / total_t / 1e6, "unit": "Mrays/s", "cores": threads,
+            "kind": "port",
+            "sample": f"FP64 oracle render of {total_rays} rays (every {stride}th-or-sparser "
+                      f"pixel of both 1920x1080 eyes), {total_t:.1f} s"}
+    return base, psnr, total_rays
+
+
+def frameprep_leg(N, n_frames=256, w=1280, h=720):
+    """Config 0 frame prep on device: 256 frames per launch, bit-exact kernel."""
+    rng = np.random.default_rng(0)
+    a = rng.integers(0, 256, (n_frames, h, w, 3), dtype=np.uint8)
+    b = rng.integers(0, 256, (n_frames, h, w, 3), dtype=np.uint8)
+    da = rng.uniform(0.1, 10.0, (n_frames, h, w)).astype(np.float32)
+    db = rng.uniform(0.1, 10.0, (n_frames, h, w)).astype(np.float32)
+    bufs = [N.DeviceBuffer(x.nbytes) for x in (a, da, b, db)]
+    for buf, x in zip(bufs, (a, da, b, db)):
+        buf.upload(x)
+    Y = N.DeviceBuffer(n_frames * w * h)
+    U = N.DeviceBuffer(n_frames * w * h // 4)
+    V = N.DeviceBuffer(n_frames * w * h // 4)
+    N.frameprep_device(*bufs, w, h, n_frames, Y, U, V)
+    ms = sorted(N.frameprep_device(*bufs, w, h, n_frames, Y, U, V) for _ in range(10))[5]
+    for buf in bufs + [Y, U, V]:
+        buf.free()
+    bytes_per_frame = w * h * 15.5
+    return {"frames_per_launch": n_frames, "ms_per_launch": ms,
+            "frames_per_s": n_frames / ms * 1e3,
+            "achieved_GBs": bytes_per_frame * n_frames / ms / 1e6,
+            "hbm_frac": bytes_per_frame * n_frames / ms / 1e6 / PEAK_HBM_GBS}
+
+
+def load_traffic():
+    p = os.path.join(ROOT, "profiles", "eval_mixed_stereo1080p_pmc.json")
+    if os.path.exists(p):
+        with open(p) as f:
+            return json.load(f).get("hbm_bytes_per_launch")
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-frameprep", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(see DESIGN.md for how the path would shard)", file=sys.stderr)
+        sys.exit(2)
+
+    import nes_amd as N
+    name, cus, _ = N.device_info()
+    log(f"device {name} ({cus} CUs)")
+    table, weights, occ = N.scene_synthesize(1)
+    views = stereo_views(N)
+    rays = sum(v.width * v.height for v in views)
+    with N.Context(table, weights, occ) as ctx:
+        wall_m, st_m = time_render(N, ctx, views, N.PRECISION_MIXED, args.steps, args.warmup)
+        log(f"mixed: {wall_m / args.steps * 1e3:.3f} ms/step")
+        wall_f, st_f = time_render(N, ctx, views, N.PRECISION_FP32, max(3, args.steps // 4),
+                                   args.warmup)
+        steps_f = len(st_f)
+        log(f"fp32: {wall_f / steps_f * 1e3:.3f} ms/step")
+        samples = st_m[-1].samples
+        ms_eval = float(np.mean([s.ms_eval for s in st_m]))
+        ms_march = float(np.mean([s.ms_march for s in st_m]))
+        ms_comp = float(np.mean([s.ms_composite for s in st_m]))
+        value = rays * args.steps / wall_m / 1e6
+        fp32 = rays * steps_f / wall_f / 1e6
+        achieved = FLOP_PER_SAMPLE * samples / (ms_eval * 1e-3) / 1e12
+        result = {
+            "metric": METRIC, "value": round(value, 3), "unit": "Mrays/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(wall_m / args.steps * 1e3, 4), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "fp16",
+            "data": "synthetic (seeded scene: N(0,0.1) hash table, Xavier MLPs, 30% occupancy blob)",
+            "config": {"workload": "stereo render 1920x1080 x 2 eyes (64 mm IPD), mixed "
+                                   "precision: FP16 table, FP16 MFMA MLPs, FP32 compositing",
+                       "rays_per_step": rays, "samples_per_step": int(samples),
+                       "samples_per_ray": round(samples / rays, 4)},
+            "fp32_mrays": round(fp32, 3),
+            "speedup_vs_fp32": round(value / fp32, 3),
+            "stage_ms": {"march": round(ms_march, 4), "eval": round(ms_eval, 4),
+                         "composite": round(ms_comp, 4),
+                         "fp32_eval": round(float(np.mean([s.ms_eval for s in st_f])), 4)},
+            "roofline": {"kernel": "eval_mixed", "bound": "mfma",
+                         "achieved": round(achieved, 3), "peak": PEAK_FP16_TFLOPS,
+                         "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP16_TFLOPS, 5),
+                         "traffic": load_traffic()},
+            "psnr_db": None, "p99_frame_latency_ms": None, "cpu_baseline": None,
+        }
+        if not args.skip_frameprep:
+            result["frameprep"] = frameprep_leg(N)
+        if not args.skip_cpu:
+            base, psnr, nr = cpu_baseline(N, table, weights, occ, ctx, views, args.cpu_seconds)
+            result["cpu_baseline"] = base
+            result["psnr_db"] = round(psnr, 2)
+            result["psnr_rays"] = nr
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

@@ -156,6 +156,83 @@ int nes_gray8_to_yuv420p(const uint8_t *gray, int w, int h, int g_stride,
                          uint8_t *y, int y_stride, uint8_t *u, int u_stride,
                          uint8_t *v, int v_stride, int sws_mode);
 
+/* --------------------------------------------------------------- serving */
+/* Continuous-batching server: replaces the reference's per-connection loop
+ * of one blocking FrameRequest in flight (socket_client_thread,
+ * src/server.cpp:116-212) plus the per-eye process_frame_thread conversion
+ * (src/encode.cpp:42-119).  Clients submit stereo requests from any thread;
+ * a scheduler batches waiting requests into free batch slots (one HIP stream
+ * and one batch of device buffers each; the HBM budget decides how many
+ * slots exist), renders, converts every view to YUV420P on the device and
+ * copies the planes to pinned host memory.  As in CameraManager
+ * (src/base/camera_manager.cc:33-69) only a client's latest pose matters: a
+ * newer request supersedes that client's request still waiting for
+ * admission (it completes with NES_ECANCELED). */
+#define NES_ECANCELED (-125)
+
+typedef struct nes_server nes_server;
+
+typedef struct nes_server_config {
+  uint64_t hbm_budget_bytes; /* 0 = 48 GiB */
+  uint32_t max_batch_rays;   /* ray slots per batch, 0 = 2^24 */
+  uint32_t max_inflight;     /* batch slots (streams), 0 = 3 */
+  uint32_t queue_capacity;   /* waiting requests, 0 = 256 */
+  int precision;             /* NES_PRECISION_* */
+  int sws_mode;              /* NES_SWS_* */
+} nes_server_config;
+
+typedef struct nes_frame_result {
+  uint64_t ticket;
+  uint64_t index;
+  uint32_t n_views;
+  uint32_t width[2], height[2];
+  /* tightly packed YUV420P planes (linesizes w and w/2), valid until
+   * nes_server_release */
+  const uint8_t *y[2];
+  const uint8_t *u[2];
+  const uint8_t *v[2];
+  double latency_ms; /* submit -> planes in host memory */
+  double queue_ms;   /* submit -> admitted into a batch */
+  uint64_t rays, samples;
+  int status;        /* NES_OK, NES_ECANCELED (superseded) or an error */
+} nes_frame_result;
+
+typedef struct nes_server_stats {
+  uint64_t completed, superseded, failed, rays, samples, batches;
+  uint32_t slots;
+  double p50_ms, p99_ms, mean_ms, max_ms;
+  double wall_s;       /* first arrival -> last completion */
+  double mrays_per_s;  /* completed rays / wall_s */
+} nes_server_stats;
+
+int nes_server_create(nes_ctx *ctx, const nes_server_config *cfg, nes_server **out);
+int nes_server_destroy(nes_server *server);
+/* n_views: 1 or 2 (stereo); even sizes.  NES_EBUSY when the queue is full. */
+int nes_server_submit(nes_server *server, uint32_t client, const nes_camera *views,
+                      int n_views, uint64_t *ticket);
+/* timeout_ms < 0 waits forever; NES_EAGAIN on timeout. */
+int nes_server_wait(nes_server *server, uint64_t ticket, int timeout_ms,
+                    nes_frame_result *out);
+int nes_server_release(nes_server *server, uint64_t ticket);
+int nes_server_stats_get(nes_server *server, nes_server_stats *stats);
+int nes_server_reset_stats(nes_server *server);
+
+/* Built-in Poisson load generator (benchmarks): n_clients clients at rate_hz
+ * each for duration_s, seeded head poses on a radius-2 orbit, FOV uniform in
+ * [fov_min, fov_max], per-eye resolution drawn from res_wh (n_res pairs). */
+typedef struct nes_sim_config {
+  uint32_t n_clients;
+  double rate_hz;
+  double duration_s;
+  uint64_t seed;
+  uint32_t n_res;
+  const uint32_t *res_wh;
+  double fov_min, fov_max;
+  int stereo;
+} nes_sim_config;
+int nes_server_simulate(nes_server *server, const nes_sim_config *sim,
+                        nes_server_stats *stats);
+
 /* ------------------------------------------------------ device memory */
 /* Minimal device-memory helpers so callers (tests, benchmarks) need no other
  * GPU runtime. */

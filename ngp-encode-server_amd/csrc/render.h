@@ -56,9 +56,23 @@ void scene_free(SceneDev &s);
 // v_mfma_f32_16x16x32_f16 instructions one 16-sample tile uses.
 void build_mlp_fragments(const float *w, _Float16 *frag);
 
-struct RenderOutputs {  // byte offsets inside RenderWork::out per pixel
-  static size_t bytes_per_px() { return 3 + 4 + 12 + 2; }
+// Byte offsets of the per-pixel outputs inside RenderWork::out.
+struct OutLayout {
+  size_t rgb24, depth, rgbf, nsamp, total;
 };
+OutLayout out_layout(size_t px);
+
+// Stages of one render (single pass: k0 = 0, k1 = 128, first = last = true):
+//   render_setup        views -> ray-slot layout, buffers, view upload
+//   render_launch_count march_count + async read-back of the sample total
+//                       into work.total_host[0]
+//   render_launch_rest  (after the total is known) march_fill, eval, composite
+int render_setup(RenderWork &w, const nes_camera *views, int n_views, hipStream_t stream);
+int render_launch_count(const SceneDev &scene, RenderWork &w, int k0, int k1, bool first,
+                        hipStream_t stream);
+int render_launch_rest(const SceneDev &scene, RenderWork &w, int precision, int k0, int k1,
+                       bool last, uint32_t n, hipStream_t stream, hipEvent_t ev_eval_begin,
+                       hipEvent_t ev_eval_end);
 
 // Enqueue a full render of views on stream; returns after the sample count
 // read-backs (one host sync per pass).  Outputs land in work.out.

@@ -301,7 +301,122 @@ __device__ __forceinline__ void sample_geometry(const uint32_t *__restrict__ sam
   }
 }
 
+// ---------------------------------------------------- level-split encode
+// A wavefront encodes 16 samples at a time: lane (g = lane / 16, j = lane % 16)
+// computes levels 4g .. 4g+3 of sample j, i.e. 32 independent corner gathers
+// per lane.  Each lane always serves the same four levels, so their constants
+// live in registers and every lane runs the same instruction stream (dense and
+// hashed indices are both formed and selected per lane).  The eight features
+// a lane produces are exactly the K-slots (g, 0..7) of the first MFMA's B
+// operand for sample j.
+struct LevelParams {
+  float scale[4];
+  uint32_t n1[4], n1sq[4], off[4];
+  bool dense[4];
+};
+
+__device__ __forceinline__ LevelParams level_params(int g) {
+  LevelParams p;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int l = 4 * g + q;
+    // kLevelRes / kLevelOffset are constexpr; a 4-way select keeps them out
+    // of memory
+    int res = 0;
+    uint32_t off = 0;
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      if (g == c) {
+        res = kLevelRes[4 * c + q];
+        off = kLevelOffset[4 * c + q];
+      }
+    p.scale[q] = (float)res;
+    p.n1[q] = (uint32_t)res + 1u;
+    p.n1sq[q] = p.n1[q] * p.n1[q];
+    p.off[q] = off;
+    p.dense[q] = l < kDenseLevels;
+  }
+  return p;
+}
+
+__device__ __forceinline__ uint32_t corner_entry(const LevelParams &p, int q, uint32_t x,
+                                                 uint32_t y, uint32_t z) {
+  const uint32_t dense = x + y * p.n1[q] + z * p.n1sq[q];
+  const uint32_t hash = (x ^ (y * kPrime1) ^ (z * kPrime2)) & ((1u << kLog2T) - 1u);
+  return p.off[q] + (p.dense[q] ? dense : hash);
+}
+
+template <class Tab> struct RawFeat;
+template <> struct RawFeat<float> {
+  typedef float2 type;
+  static __device__ __forceinline__ float2 load(const float *__restrict__ t, uint32_t e) {
+    return __ldg(reinterpret_cast<const float2 *>(t) + e);
+  }
+  static __device__ __forceinline__ float2 unpack(float2 v) { return v; }
+};
+template <> struct RawFeat<_Float16> {
+  typedef uint32_t type;
+  static __device__ __forceinline__ uint32_t load(const _Float16 *__restrict__ t, uint32_t e) {
+    return __ldg(reinterpret_cast<const uint32_t *>(t) + e);
+  }
+  static __device__ __forceinline__ float2 unpack(uint32_t bits) {
+    const half2v h = __builtin_bit_cast(half2v, bits);
+    return make_float2((float)h.x, (float)h.y);
+  }
+};
+
+// Eight features (levels 4g..4g+3, two each) of one sample, FP32.
+template <class Tab>
+__device__ __forceinline__ void encode4(const Tab *__restrict__ table, const LevelParams &lp,
+                                        const float xc[3], float f[8]) {
+  uint32_t e[4][8];
+  float fr[4][3];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    uint32_t i0[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const float pp = xc[a] * lp.scale[q];
+      const float fl = fminf(floorf(pp), lp.scale[q] - 1.0f);
+      i0[a] = (uint32_t)fl;
+      fr[q][a] = pp - fl;
+    }
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+      e[q][c] = corner_entry(lp, q, i0[0] + (c & 1), i0[1] + ((c >> 1) & 1), i0[2] + ((c >> 2) & 1));
+  }
+  // raw words stay packed until they are consumed (1 VGPR per FP16 corner)
+  typename RawFeat<Tab>::type v[4][8];
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int c = 0; c < 8; ++c) v[q][c] = RawFeat<Tab>::load(table, e[q][c]);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const float gx = 1.0f - fr[q][0], gy = 1.0f - fr[q][1], gz = 1.0f - fr[q][2];
+    float a0 = 0.0f, a1 = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const float w = ((c & 1) ? fr[q][0] : gx) * (((c >> 1) & 1) ? fr[q][1] : gy) *
+                      (((c >> 2) & 1) ? fr[q][2] : gz);
+      const float2 fv = RawFeat<Tab>::unpack(v[q][c]);
+      a0 = fmaf(w, fv.x, a0);
+      a1 = fmaf(w, fv.y, a1);
+    }
+    f[2 * q] = a0;
+    f[2 * q + 1] = a1;
+  }
+}
+
 __device__ __forceinline__ float sigmoidf_(float v) { return 1.0f / (1.0f + expf(-v)); }
+
+// Orders a wavefront's own LDS writes before its later LDS reads of other
+// lanes' data (LDS ops of one wavefront complete in order).
+__device__ __forceinline__ void wave_lds_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 // ------------------------------------------------------------ FP32 eval
 template <int NI, int NO, bool kRelu>
@@ -317,34 +432,56 @@ __device__ __forceinline__ void dense_f32(const float *w, const float *in, float
   }
 }
 
-// One lane per sample; the 9344 weights sit in LDS and are read as
-// wave-uniform broadcasts.
+// 64 samples per wavefront iteration: four level-split encode passes of 16
+// samples fill an LDS feature block, then each lane runs one sample's MLPs in
+// FP32 with the 9344 weights read from LDS as wave-uniform broadcasts.
+constexpr int kEvalWaves = 4;
+
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void eval_fp32(
     const uint32_t *__restrict__ samples, uint32_t n,
     const RayData *__restrict__ rays, const float *__restrict__ table,
     const float *__restrict__ Wg, float4 *__restrict__ out) {
   __shared__ __attribute__((aligned(16))) float W[kWeights];
+  __shared__ __attribute__((aligned(16))) float feat_lds[kEvalWaves][64][33];
   for (int i = (int)threadIdx.x; i < kWeights / 4; i += (int)blockDim.x)
     reinterpret_cast<float4 *>(W)[i] = reinterpret_cast<const float4 *>(Wg)[i];
   __syncthreads();
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += gridDim.x * blockDim.x) {
-    float xc[3], d[3];
-    sample_geometry(samples, rays, i, xc, d);
-    float feat[32];
-    encode(table, xc, feat);
-    float h1[64], o2[16], in3[31], h3[64], h4[64], o5[3], sh[16];
-    dense_f32<32, 64, true>(W + kW1, feat, h1);
-    dense_f32<64, 16, false>(W + kW2, h1, o2);
-    sh4(d, sh);
+  const int lane = lane_id(), wave = (int)(threadIdx.x >> 6);
+  const int j = lane & 15, g = lane >> 4;
+  const LevelParams lp = level_params(g);
+  const uint32_t n_chunks = (n + 63) / 64;
+  for (uint32_t chunk = blockIdx.x * kEvalWaves + wave; chunk < n_chunks;
+       chunk += gridDim.x * kEvalWaves) {
+#pragma unroll 1
+    for (int t = 0; t < 4; ++t) {
+      const uint32_t s = min(chunk * 64 + 16 * t + j, n - 1);
+      float xc[3], d[3], f[8];
+      sample_geometry(samples, rays, s, xc, d);
+      encode4(table, lp, xc, f);
 #pragma unroll
-    for (int j = 0; j < 16; ++j) in3[j] = sh[j];
+      for (int e = 0; e < 8; ++e) feat_lds[wave][16 * t + j][8 * g + e] = f[e];
+    }
+    wave_lds_fence();
+    const uint32_t i = chunk * 64 + (uint32_t)lane;
+    if (i < n) {
+      float xc[3], d[3], feat[32];
+      sample_geometry(samples, rays, i, xc, d);
 #pragma unroll
-    for (int j = 0; j < 15; ++j) in3[16 + j] = o2[1 + j];
-    dense_f32<31, 64, true>(W + kW3, in3, h3);
-    dense_f32<64, 64, true>(W + kW4, h3, h4);
-    dense_f32<64, 3, false>(W + kW5, h4, o5);
-    out[i] = make_float4(expf(o2[0]), sigmoidf_(o5[0]), sigmoidf_(o5[1]), sigmoidf_(o5[2]));
+      for (int e = 0; e < 32; ++e) feat[e] = feat_lds[wave][lane][e];
+      float h1[64], o2[16], in3[31], h3[64], h4[64], o5[3], sh[16];
+      dense_f32<32, 64, true>(W + kW1, feat, h1);
+      dense_f32<64, 16, false>(W + kW2, h1, o2);
+      sh4(d, sh);
+#pragma unroll
+      for (int k = 0; k < 16; ++k) in3[k] = sh[k];
+#pragma unroll
+      for (int k = 0; k < 15; ++k) in3[16 + k] = o2[1 + k];
+      dense_f32<31, 64, true>(W + kW3, in3, h3);
+      dense_f32<64, 64, true>(W + kW4, h3, h4);
+      dense_f32<64, 3, false>(W + kW5, h4, o5);
+      out[i] = make_float4(expf(o2[0]), sigmoidf_(o5[0]), sigmoidf_(o5[1]), sigmoidf_(o5[2]));
+    }
+    wave_lds_fence();
   }
 }
 
@@ -355,10 +492,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void e
 // A neuron block's D therefore already sits where the next layer's B needs it
 // once the K index of slot (g, s) is taken to be
 //   s < 4 : block 2c   neuron 4g + s        s >= 4 : block 2c+1 neuron 4g + s-4
-// which the host bakes into the weight fragments (build_mlp_fragments).
-constexpr int kEvalWaves = 4;
-constexpr int kLdsRow = 48;  // 32 features + 16 SH (halves)
-
+// which the host bakes into the weight fragments (build_mlp_fragments).  The
+// level-split encode delivers layer 1's B operand straight from the gathers.
 __device__ __forceinline__ float4v mfma(half8 a, half8 b, float4v c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
 }
@@ -372,96 +507,68 @@ __device__ __forceinline__ half8 pack_relu(float4v lo, float4v hi) {
   return r;
 }
 
-__device__ __forceinline__ void wave_lds_fence() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-template <class Tab, int L = 0>
-__device__ __forceinline__ void encode_to_lds(const Tab *__restrict__ table,
-                                              const float xc[3], _Float16 *row) {
-  if constexpr (L < kLevels) {
-    float f0, f1;
-    encode_level<L>(table, xc, f0, f1);
-    half2v h;
-    h.x = (_Float16)f0;
-    h.y = (_Float16)f1;
-    *reinterpret_cast<half2v *>(row + 2 * L) = h;
-    encode_to_lds<Tab, L + 1>(table, xc, row);
+// SH components 4g .. 4g+3 of a unit direction.
+__device__ __forceinline__ half4 sh4_quarter(const float d[3], int g) {
+  float sh[16];
+  sh4(d, sh);
+  half4 r;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    float v = sh[k];
+#pragma unroll
+    for (int c = 1; c < 4; ++c)
+      if (g == c) v = sh[4 * c + k];
+    r[k] = (_Float16)v;
   }
+  return r;
 }
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void eval_mixed(
+__global__ __launch_bounds__(256) void eval_mixed(
     const uint32_t *__restrict__ samples, uint32_t n,
     const RayData *__restrict__ rays, const _Float16 *__restrict__ table,
     const half8 *__restrict__ wfrag, float4 *__restrict__ out) {
-  __shared__ __attribute__((aligned(16))) _Float16 lds[kEvalWaves][64 * kLdsRow];
   __shared__ half8 wl[20 * 64];
   for (int i = (int)threadIdx.x; i < 20 * 64; i += (int)blockDim.x) wl[i] = wfrag[i];
   __syncthreads();
-  const int lane = lane_id();
-  const int wave = (int)(threadIdx.x >> 6);
-  _Float16 *row_base = lds[wave];
-  const half8 *wf = wl + lane;  // fragment m at wf[64 * m]
+  const int lane = lane_id(), wave = (int)(threadIdx.x >> 6);
   const int j = lane & 15, g = lane >> 4;
+  const half8 *wf = wl + lane;  // fragment m at wf[64 * m]
   const float4v zero = {0.0f, 0.0f, 0.0f, 0.0f};
-
-  const uint32_t n_chunks = (n + 63) / 64;
-  for (uint32_t chunk = blockIdx.x * kEvalWaves + wave; chunk < n_chunks;
-       chunk += gridDim.x * kEvalWaves) {
-    const uint32_t i = chunk * 64 + (uint32_t)lane;
-    // Encode this lane's sample into its LDS row.
-    _Float16 *row = row_base + lane * kLdsRow;
-    if (i < n) {
-      float xc[3], d[3], sh[16];
-      sample_geometry(samples, rays, i, xc, d);
-      encode_to_lds(table, xc, row);
-      sh4(d, sh);
+  const LevelParams lp = level_params(g);
+  const uint32_t n_tiles = (n + 15) / 16;
+  for (uint32_t tile = blockIdx.x * kEvalWaves + wave; tile < n_tiles;
+       tile += gridDim.x * kEvalWaves) {
+    const uint32_t s = tile * 16 + (uint32_t)j;
+    float xc[3], d[3], f[8];
+    sample_geometry(samples, rays, min(s, n - 1), xc, d);
+    encode4(table, lp, xc, f);
+    half8 b1;
 #pragma unroll
-      for (int q = 0; q < 2; ++q) {
-        half8 h;
+    for (int e = 0; e < 8; ++e) b1[e] = (_Float16)f[e];
+    const half4 shv = sh4_quarter(d, g);
+    float4v d1[4];
 #pragma unroll
-        for (int e = 0; e < 8; ++e) h[e] = (_Float16)sh[8 * q + e];
-        *reinterpret_cast<half8 *>(row + 32 + 8 * q) = h;
-      }
-    } else {
-      const half8 hz = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int m = 0; m < 4; ++m) d1[m] = mfma(wf[64 * m], b1, zero);
+    float4v d2 = mfma(wf[64 * 4], pack_relu(d1[0], d1[1]), zero);
+    d2 = mfma(wf[64 * 5], pack_relu(d1[2], d1[3]), d2);
+    half8 b3;
+    b3[0] = (_Float16)d2[0]; b3[1] = (_Float16)d2[1];
+    b3[2] = (_Float16)d2[2]; b3[3] = (_Float16)d2[3];
+    b3[4] = shv[0]; b3[5] = shv[1]; b3[6] = shv[2]; b3[7] = shv[3];
+    float4v d3[4];
 #pragma unroll
-      for (int q = 0; q < 6; ++q) *reinterpret_cast<half8 *>(row + 8 * q) = hz;
+    for (int m = 0; m < 4; ++m) d3[m] = mfma(wf[64 * (6 + m)], b3, zero);
+    const half8 b4a = pack_relu(d3[0], d3[1]), b4b = pack_relu(d3[2], d3[3]);
+    float4v d4[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      d4[m] = mfma(wf[64 * (10 + 2 * m)], b4a, zero);
+      d4[m] = mfma(wf[64 * (11 + 2 * m)], b4b, d4[m]);
     }
-    wave_lds_fence();
-#pragma unroll 1
-    for (int t = 0; t < 4; ++t) {
-      const _Float16 *src = row_base + (16 * t + j) * kLdsRow;
-      const half8 b1 = *reinterpret_cast<const half8 *>(src + 8 * g);
-      float4v d1[4];
-#pragma unroll
-      for (int m = 0; m < 4; ++m) d1[m] = mfma(wf[64 * m], b1, zero);
-      float4v d2 = mfma(wf[64 * 4], pack_relu(d1[0], d1[1]), zero);
-      d2 = mfma(wf[64 * 5], pack_relu(d1[2], d1[3]), d2);
-      const half4 shv = *reinterpret_cast<const half4 *>(src + 32 + 4 * g);
-      half8 b3;
-      b3[0] = (_Float16)d2[0]; b3[1] = (_Float16)d2[1];
-      b3[2] = (_Float16)d2[2]; b3[3] = (_Float16)d2[3];
-      b3[4] = shv[0]; b3[5] = shv[1]; b3[6] = shv[2]; b3[7] = shv[3];
-      float4v d3[4];
-#pragma unroll
-      for (int m = 0; m < 4; ++m) d3[m] = mfma(wf[64 * (6 + m)], b3, zero);
-      const half8 b4a = pack_relu(d3[0], d3[1]), b4b = pack_relu(d3[2], d3[3]);
-      float4v d4[4];
-#pragma unroll
-      for (int m = 0; m < 4; ++m) {
-        d4[m] = mfma(wf[64 * (10 + 2 * m)], b4a, zero);
-        d4[m] = mfma(wf[64 * (11 + 2 * m)], b4b, d4[m]);
-      }
-      float4v d5 = mfma(wf[64 * 18], pack_relu(d4[0], d4[1]), zero);
-      d5 = mfma(wf[64 * 19], pack_relu(d4[2], d4[3]), d5);
-      const uint32_t s = chunk * 64 + 16 * t + j;
-      if (g == 0 && s < n)
-        out[s] = make_float4(expf(d2[0]), sigmoidf_(d5[0]), sigmoidf_(d5[1]), sigmoidf_(d5[2]));
-    }
-    wave_lds_fence();
+    float4v d5 = mfma(wf[64 * 18], pack_relu(d4[0], d4[1]), zero);
+    d5 = mfma(wf[64 * 19], pack_relu(d4[2], d4[3]), d5);
+    if (g == 0 && s < n)
+      out[s] = make_float4(expf(d2[0]), sigmoidf_(d5[0]), sigmoidf_(d5[1]), sigmoidf_(d5[2]));
   }
 }
 
@@ -603,10 +710,7 @@ void render_work_free(RenderWork &w) {
 
 static size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
-struct OutLayout {
-  size_t rgb24, depth, rgbf, nsamp, total;
-};
-static OutLayout out_layout(size_t px) {
+OutLayout out_layout(size_t px) {
   OutLayout l;
   l.rgb24 = 0;
   l.depth = align_up(3 * px, 256);
@@ -616,13 +720,18 @@ static OutLayout out_layout(size_t px) {
   return l;
 }
 
-int render_enqueue(const SceneDev &scene, RenderWork &w, int precision,
-                   const nes_camera *views, int n_views, int steps_per_pass,
-                   hipStream_t stream, nes_render_stats *stats) {
+static OutPtrs out_ptrs(RenderWork &w) {
+  const OutLayout L = out_layout(w.out_px);
+  OutPtrs op;
+  op.rgb24 = w.out.as<uint8_t>() + L.rgb24;
+  op.depth = reinterpret_cast<float *>(w.out.as<uint8_t>() + L.depth);
+  op.rgbf = reinterpret_cast<float *>(w.out.as<uint8_t>() + L.rgbf);
+  op.nsamp = reinterpret_cast<uint16_t *>(w.out.as<uint8_t>() + L.nsamp);
+  return op;
+}
+
+int render_setup(RenderWork &w, const nes_camera *views, int n_views, hipStream_t stream) {
   if (!views || n_views < 1) return NES_EINVAL;
-  if (precision != NES_PRECISION_FP32 && precision != NES_PRECISION_MIXED) return NES_EINVAL;
-  if (steps_per_pass < 0) return NES_EINVAL;
-  const int spp = steps_per_pass == 0 ? kMaxSteps : std::min(steps_per_pass, kMaxSteps);
   w.host_views.resize((size_t)n_views);
   uint64_t slots = 0, px = 0;
   for (int i = 0; i < n_views; ++i) {
@@ -646,84 +755,103 @@ int render_enqueue(const SceneDev &scene, RenderWork &w, int precision,
     v.pad = 0;
     slots += v.slot_count;
     px += (uint64_t)c.width * c.height;
-  }
-  if (slots > kMaxSlots) {
-    set_last_error("render batch exceeds 2^25 ray slots");
-    return NES_ERANGE;
+    if (slots > kMaxSlots) {
+      set_last_error("render batch exceeds 2^25 ray slots");
+      return NES_ERANGE;
+    }
   }
   w.n_views = (uint32_t)n_views;
   w.n_slots = (uint32_t)slots;
   w.out_px = px;
-  const OutLayout L = out_layout(px);
   NES_TRY(w.views.reserve(sizeof(ViewDesc) * n_views));
   NES_TRY(w.rays.reserve(sizeof(RayData) * slots));
   NES_TRY(w.acc.reserve(sizeof(RayAcc) * slots));
   NES_TRY(w.counts.reserve(sizeof(uint32_t) * slots));
   NES_TRY(w.offsets.reserve(sizeof(uint32_t) * slots));
-  NES_TRY(w.out.reserve(L.total));
+  NES_TRY(w.out.reserve(out_layout(px).total));
   NES_TRY(w.total.reserve(64));
   if (!w.total_host) NES_HIP_TRY(hipHostMalloc((void **)&w.total_host, 64));
   for (hipEvent_t &e : w.ev)
     if (!e) NES_HIP_TRY(hipEventCreate(&e));
-
   NES_HIP_TRY(hipMemcpyAsync(w.views.ptr, w.host_views.data(), sizeof(ViewDesc) * n_views,
                              hipMemcpyHostToDevice, stream));
-  const unsigned blocks = (unsigned)((slots + 255) / 256);
-  uint32_t *total = w.total.as<uint32_t>();
-  OutPtrs op;
-  op.rgb24 = w.out.as<uint8_t>() + L.rgb24;
-  op.depth = reinterpret_cast<float *>(w.out.as<uint8_t>() + L.depth);
-  op.rgbf = reinterpret_cast<float *>(w.out.as<uint8_t>() + L.rgbf);
-  op.nsamp = reinterpret_cast<uint16_t *>(w.out.as<uint8_t>() + L.nsamp);
+  return NES_OK;
+}
 
-  int dev_cu = 256;
-  double ms_march = 0, ms_eval = 0, ms_comp = 0, ms_total = 0;
+int render_launch_count(const SceneDev &scene, RenderWork &w, int k0, int k1, bool first,
+                        hipStream_t stream) {
+  const unsigned blocks = (unsigned)((w.n_slots + 255) / 256);
+  uint32_t *total = w.total.as<uint32_t>();
+  zero_u32<<<1, 64, 0, stream>>>(total, 2);
+  march_count<<<blocks, 256, 0, stream>>>(w.views.as<ViewDesc>(), w.n_views, w.n_slots, scene.occ,
+                                         w.rays.as<RayData>(), w.acc.as<RayAcc>(),
+                                         w.counts.as<uint32_t>(), total, k0, k1, first ? 1 : 0);
+  NES_HIP_TRY(hipGetLastError());
+  NES_HIP_TRY(hipMemcpyAsync(w.total_host, total, 4, hipMemcpyDeviceToHost, stream));
+  return NES_OK;
+}
+
+int render_launch_rest(const SceneDev &scene, RenderWork &w, int precision, int k0, int k1,
+                       bool last, uint32_t n, hipStream_t stream, hipEvent_t ev_eval_begin,
+                       hipEvent_t ev_eval_end) {
+  const unsigned blocks = (unsigned)((w.n_slots + 255) / 256);
+  uint32_t *total = w.total.as<uint32_t>();
+  NES_TRY(w.samples.reserve(sizeof(uint32_t) * ((size_t)n + 1)));
+  NES_TRY(w.sample_out.reserve(sizeof(float4) * ((size_t)n + 1)));
+  march_fill<<<blocks, 256, 0, stream>>>(w.n_slots, scene.occ, w.rays.as<RayData>(),
+                                        w.counts.as<uint32_t>(), w.offsets.as<uint32_t>(),
+                                        w.samples.as<uint32_t>(), total + 1, k0, k1);
+  NES_HIP_TRY(hipGetLastError());
+  if (ev_eval_begin) NES_HIP_TRY(hipEventRecord(ev_eval_begin, stream));
+  if (n) {
+    const int dev_cu = 256;
+    if (precision == NES_PRECISION_FP32) {
+      const uint64_t chunks = (n + 63) / 64;
+      const unsigned eb = (unsigned)std::min<uint64_t>((chunks + kEvalWaves - 1) / kEvalWaves,
+                                                       (uint64_t)dev_cu * 4);
+      eval_fp32<<<eb, 256, 0, stream>>>(w.samples.as<uint32_t>(), n, w.rays.as<RayData>(),
+                                        scene.table_f32, scene.weights_f32,
+                                        w.sample_out.as<float4>());
+    } else {
+      const uint64_t tiles = (n + 15) / 16;
+      const unsigned eb = (unsigned)std::min<uint64_t>((tiles + kEvalWaves - 1) / kEvalWaves,
+                                                       (uint64_t)dev_cu * 8);
+      eval_mixed<<<eb, 256, 0, stream>>>(w.samples.as<uint32_t>(), n, w.rays.as<RayData>(),
+                                         scene.table_f16,
+                                         reinterpret_cast<const half8 *>(scene.wfrag),
+                                         w.sample_out.as<float4>());
+    }
+    NES_HIP_TRY(hipGetLastError());
+  }
+  if (ev_eval_end) NES_HIP_TRY(hipEventRecord(ev_eval_end, stream));
+  composite<<<blocks, 256, 0, stream>>>(w.n_slots, w.rays.as<RayData>(), w.acc.as<RayAcc>(),
+                                       w.counts.as<uint32_t>(), w.offsets.as<uint32_t>(),
+                                       w.samples.as<uint32_t>(), w.sample_out.as<float4>(),
+                                       last ? 1 : 0, out_ptrs(w));
+  NES_HIP_TRY(hipGetLastError());
+  return NES_OK;
+}
+
+int render_enqueue(const SceneDev &scene, RenderWork &w, int precision,
+                   const nes_camera *views, int n_views, int steps_per_pass,
+                   hipStream_t stream, nes_render_stats *stats) {
+  if (precision != NES_PRECISION_FP32 && precision != NES_PRECISION_MIXED) return NES_EINVAL;
+  if (steps_per_pass < 0) return NES_EINVAL;
+  const int spp = steps_per_pass == 0 ? kMaxSteps : std::min(steps_per_pass, kMaxSteps);
+  NES_TRY(render_setup(w, views, n_views, stream));
+  double ms_march = 0, ms_eval = 0, ms_comp = 0;
   uint64_t n_samples = 0;
   NES_HIP_TRY(hipEventRecord(w.ev[0], stream));
   const int passes = (kMaxSteps + spp - 1) / spp;
   for (int p = 0; p < passes; ++p) {
     const int k0 = p * spp, k1 = std::min(kMaxSteps, k0 + spp);
-    const bool last = p == passes - 1;
-    zero_u32<<<1, 64, 0, stream>>>(total, 2);
     NES_HIP_TRY(hipEventRecord(w.ev[1], stream));
-    march_count<<<blocks, 256, 0, stream>>>(w.views.as<ViewDesc>(), w.n_views, w.n_slots, scene.occ,
-                                           w.rays.as<RayData>(), w.acc.as<RayAcc>(),
-                                           w.counts.as<uint32_t>(), total, k0, k1, p == 0);
-    NES_HIP_TRY(hipGetLastError());
-    NES_HIP_TRY(hipMemcpyAsync(w.total_host, total, 4, hipMemcpyDeviceToHost, stream));
+    NES_TRY(render_launch_count(scene, w, k0, k1, p == 0, stream));
     NES_HIP_TRY(hipStreamSynchronize(stream));
     const uint32_t n = w.total_host[0];
     n_samples += n;
-    NES_TRY(w.samples.reserve(sizeof(uint32_t) * ((size_t)n + 1)));
-    NES_TRY(w.sample_out.reserve(sizeof(float4) * ((size_t)n + 1)));
-    march_fill<<<blocks, 256, 0, stream>>>(w.n_slots, scene.occ, w.rays.as<RayData>(),
-                                          w.counts.as<uint32_t>(), w.offsets.as<uint32_t>(),
-                                          w.samples.as<uint32_t>(), total + 1, k0, k1);
-    NES_HIP_TRY(hipGetLastError());
-    NES_HIP_TRY(hipEventRecord(w.ev[2], stream));
-    if (n) {
-      if (precision == NES_PRECISION_FP32) {
-        const unsigned eb = (unsigned)std::min<uint64_t>((n + 255) / 256, (uint64_t)dev_cu * 16);
-        eval_fp32<<<eb, 256, 0, stream>>>(w.samples.as<uint32_t>(), n, w.rays.as<RayData>(),
-                                          scene.table_f32, scene.weights_f32,
-                                          w.sample_out.as<float4>());
-      } else {
-        const uint64_t chunks = (n + 63) / 64;
-        const unsigned eb = (unsigned)std::min<uint64_t>((chunks + kEvalWaves - 1) / kEvalWaves,
-                                                         (uint64_t)dev_cu * 8);
-        eval_mixed<<<eb, 256, 0, stream>>>(w.samples.as<uint32_t>(), n, w.rays.as<RayData>(),
-                                           scene.table_f16,
-                                           reinterpret_cast<const half8 *>(scene.wfrag),
-                                           w.sample_out.as<float4>());
-      }
-      NES_HIP_TRY(hipGetLastError());
-    }
-    NES_HIP_TRY(hipEventRecord(w.ev[3], stream));
-    composite<<<blocks, 256, 0, stream>>>(w.n_slots, w.rays.as<RayData>(), w.acc.as<RayAcc>(),
-                                         w.counts.as<uint32_t>(), w.offsets.as<uint32_t>(),
-                                         w.samples.as<uint32_t>(), w.sample_out.as<float4>(),
-                                         last ? 1 : 0, op);
-    NES_HIP_TRY(hipGetLastError());
+    NES_TRY(render_launch_rest(scene, w, precision, k0, k1, p == passes - 1, n, stream, w.ev[2],
+                               w.ev[3]));
     NES_HIP_TRY(hipEventRecord(w.ev[4], stream));
     NES_HIP_TRY(hipEventSynchronize(w.ev[4]));
     float a = 0, b = 0, c = 0;
@@ -736,11 +864,10 @@ int render_enqueue(const SceneDev &scene, RenderWork &w, int precision,
   }
   float t = 0;
   hipEventElapsedTime(&t, w.ev[0], w.ev[4]);
-  ms_total = t;
   if (stats) {
-    stats->rays = px;
+    stats->rays = w.out_px;
     stats->samples = n_samples;
-    stats->ms_total = ms_total;
+    stats->ms_total = t;
     stats->ms_eval = ms_eval;
     stats->ms_march = ms_march;
     stats->ms_composite = ms_comp;

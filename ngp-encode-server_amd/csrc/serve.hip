@@ -1,0 +1,588 @@
+// serve.hip -- continuous-batching render server (config: several VR clients,
+// mixed per-eye resolutions, YUV420P frames out).
+//
+// Replaces the reference's request loop: one socket_client_thread per
+// renderer connection, each with a single blocking FrameRequest in flight
+// (src/server.cpp:116-212), feeding process_frame_thread's RGB24 -> YUV420P
+// conversion (src/encode.cpp:42-119).  Here:
+//   * any number of client threads submit stereo requests (nes_server_submit);
+//     like CameraManager (src/base/camera_manager.cc:33-69), only a client's
+//     latest pose matters: a newer request supersedes one still waiting for
+//     admission;
+//   * one scheduler thread forms batches of waiting requests as soon as a
+//     batch slot is free.  A slot owns one HIP stream and the device buffers
+//     of one batch; the number of slots is what the HBM budget affords, so
+//     admission is by memory.  Batches on different slots overlap on the
+//     device, and a request arriving while others render joins the next free
+//     slot without waiting for them (continuous batching);
+//   * each batch runs the render stages (march count -> async total read-back
+//     -> fill / eval / composite) then frame prep of every view straight from
+//     the device RGB24 (no host round trip) and an async copy of the YUV420P
+//     planes into pinned host memory.  The scheduler polls events, so it never
+//     blocks on one batch while another could progress.
+// All per-ray work is independent of batch composition, so a request's frame
+// is bit-identical alone or in any mix.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <thread>
+
+#include "render.h"
+
+namespace nes {
+
+using Clock = std::chrono::steady_clock;
+
+static double ms_between(Clock::time_point a, Clock::time_point b) {
+  return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+constexpr int kMaxViews = 2;
+constexpr int NES_ECANCELED_ = NES_ECANCELED;
+
+struct PinnedBlock {
+  uint8_t *ptr = nullptr;
+  size_t bytes = 0;
+};
+
+// Pinned host buffers recycled by exact size (frame sizes repeat).
+class PinnedPool {
+ public:
+  ~PinnedPool() {
+    for (auto &kv : free_)
+      for (uint8_t *p : kv.second) hipHostFree(p);
+  }
+  int get(size_t bytes, PinnedBlock *out) {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      auto &v = free_[bytes];
+      if (!v.empty()) {
+        out->ptr = v.back();
+        out->bytes = bytes;
+        v.pop_back();
+        return NES_OK;
+      }
+    }
+    NES_HIP_TRY(hipHostMalloc((void **)&out->ptr, bytes));
+    out->bytes = bytes;
+    return NES_OK;
+  }
+  void put(PinnedBlock b) {
+    if (!b.ptr) return;
+    std::lock_guard<std::mutex> lk(mu_);
+    free_[b.bytes].push_back(b.ptr);
+  }
+
+ private:
+  std::mutex mu_;
+  std::map<size_t, std::vector<uint8_t *>> free_;
+};
+
+enum class ReqState { kWaiting, kRunning, kDone, kSuperseded, kFailed };
+
+struct Request {
+  uint64_t ticket = 0;
+  uint32_t client = 0;
+  int n_views = 0;
+  nes_camera views[kMaxViews];
+  Clock::time_point arrival, admitted, done;
+  ReqState state = ReqState::kWaiting;
+  int status = NES_OK;
+  PinnedBlock yuv;  // per view: Y then U then V, tightly packed
+  size_t view_off[kMaxViews] = {};
+  uint64_t rays = 0, samples = 0;
+};
+
+static size_t yuv_bytes(const nes_camera &c) {
+  return (size_t)c.width * c.height + 2 * (size_t)(c.width / 2) * (c.height / 2);
+}
+
+struct Slot {
+  hipStream_t stream = nullptr;
+  RenderWork work;
+  DevBuf yuv;
+  hipEvent_t ev_count = nullptr, ev_done = nullptr;
+  std::vector<std::shared_ptr<Request>> reqs;
+  std::vector<nes_camera> views;
+  int stage = 0;  // 0 idle, 1 counting, 2 rendering
+  Clock::time_point launched;
+};
+
+}  // namespace nes
+
+using namespace nes;
+
+struct nes_server {
+  nes_ctx *ctx = nullptr;
+  nes_server_config cfg{};
+  std::vector<std::unique_ptr<Slot>> slots;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<std::shared_ptr<Request>> waiting;
+  std::map<uint64_t, std::shared_ptr<Request>> tickets;
+  std::map<uint32_t, std::shared_ptr<Request>> client_waiting;
+  uint64_t next_ticket = 1;
+  bool stop = false;
+  std::thread scheduler;
+  PinnedPool pinned;
+  // statistics
+  std::vector<double> latencies_ms;
+  uint64_t completed = 0, superseded = 0, failed = 0, rays = 0, samples = 0, batches = 0;
+  Clock::time_point first_arrival, last_done;
+  bool any_arrival = false;
+  int last_error = NES_OK;
+};
+
+namespace nes {
+
+static size_t slot_bytes(uint32_t max_rays, uint32_t samples_per_ray) {
+  // ray state + counts/offsets + per-pixel outputs + YUV + samples
+  const size_t r = max_rays;
+  return r * (sizeof(RayData) + sizeof(RayAcc) + 8 + 21 + 2) +
+         r * (size_t)samples_per_ray * (4 + 16) + (1u << 20);
+}
+
+// Launch frame prep of every view of a finished render and the D2H copies.
+static int launch_output(nes_server *s, Slot &sl) {
+  RenderWork &w = sl.work;
+  const OutLayout L = out_layout(w.out_px);
+  size_t yuv_total = 0;
+  for (const nes_camera &c : sl.views) yuv_total += yuv_bytes(c);
+  NES_TRY(sl.yuv.reserve(yuv_total + 256));
+  size_t off = 0;
+  size_t vi = 0;
+  for (auto &rq : sl.reqs) {
+    for (int v = 0; v < rq->n_views; ++v, ++vi) {
+      const nes_camera &c = sl.views[vi];
+      const ViewDesc &vd = w.host_views[vi];
+      const uint8_t *rgb = w.out.as<uint8_t>() + L.rgb24 + 3 * (size_t)vd.px_base;
+      uint8_t *y = sl.yuv.as<uint8_t>() + off;
+      uint8_t *u = y + (size_t)c.width * c.height;
+      uint8_t *vv = u + (size_t)(c.width / 2) * (c.height / 2);
+      NES_TRY(launch_frameprep(rgb, nullptr, nullptr, nullptr, (int)c.width, (int)c.height, 1,
+                               0, 0, y, (int)c.width, 0, u, (int)c.width / 2, 0, vv,
+                               (int)c.width / 2, 0, s->cfg.sws_mode, sl.stream));
+      NES_HIP_TRY(hipMemcpyAsync(rq->yuv.ptr + rq->view_off[v], y, yuv_bytes(c),
+                                 hipMemcpyDeviceToHost, sl.stream));
+      off += yuv_bytes(c);
+    }
+  }
+  return NES_OK;
+}
+
+// Take waiting requests (FIFO) that fit one batch.
+static void form_batch(nes_server *s, Slot &sl) {
+  sl.reqs.clear();
+  sl.views.clear();
+  uint64_t slots_used = 0;
+  while (!s->waiting.empty()) {
+    auto rq = s->waiting.front();
+    uint64_t need = 0;
+    for (int v = 0; v < rq->n_views; ++v)
+      need += (uint64_t)((rq->views[v].width + 7) / 8) * ((rq->views[v].height + 7) / 8) * 64;
+    if (!sl.reqs.empty() && slots_used + need > s->cfg.max_batch_rays) break;
+    s->waiting.pop_front();
+    auto it = s->client_waiting.find(rq->client);
+    if (it != s->client_waiting.end() && it->second == rq) s->client_waiting.erase(it);
+    rq->state = ReqState::kRunning;
+    rq->admitted = Clock::now();
+    slots_used += need;
+    for (int v = 0; v < rq->n_views; ++v) sl.views.push_back(rq->views[v]);
+    sl.reqs.push_back(rq);
+  }
+}
+
+static void finish_batch(nes_server *s, Slot &sl, int status) {
+  const Clock::time_point now = Clock::now();
+  const uint64_t n = sl.work.total_host ? sl.work.total_host[0] : 0;
+  uint64_t batch_rays = 0;
+  for (auto &rq : sl.reqs)
+    for (int v = 0; v < rq->n_views; ++v) batch_rays += (uint64_t)rq->views[v].width * rq->views[v].height;
+  std::lock_guard<std::mutex> lk(s->mu);
+  for (auto &rq : sl.reqs) {
+    rq->done = now;
+    rq->status = status;
+    rq->state = status == NES_OK ? ReqState::kDone : ReqState::kFailed;
+    uint64_t r = 0;
+    for (int v = 0; v < rq->n_views; ++v) r += (uint64_t)rq->views[v].width * rq->views[v].height;
+    rq->rays = r;
+    rq->samples = batch_rays ? (uint64_t)((double)n * (double)r / (double)batch_rays) : 0;
+    if (status == NES_OK) {
+      s->completed++;
+      s->rays += r;
+      s->latencies_ms.push_back(ms_between(rq->arrival, now));
+      s->last_done = now;
+    } else {
+      s->failed++;
+    }
+  }
+  s->samples += n;
+  s->batches++;
+  sl.reqs.clear();
+  sl.stage = 0;
+  s->cv_done.notify_all();
+}
+
+static void scheduler_loop(nes_server *s) {
+  hipSetDevice(0);
+  const SceneDev &scene = s->ctx->scene;
+  while (true) {
+    bool progressed = false;
+    for (auto &slp : s->slots) {
+      Slot &sl = *slp;
+      if (sl.stage == 1 && hipEventQuery(sl.ev_count) == hipSuccess) {
+        const uint32_t n = sl.work.total_host[0];
+        int rc = render_launch_rest(scene, sl.work, s->cfg.precision, 0, kMaxSteps, true, n,
+                                    sl.stream, nullptr, nullptr);
+        if (rc == NES_OK) rc = launch_output(s, sl);
+        if (rc == NES_OK && hipEventRecord(sl.ev_done, sl.stream) != hipSuccess) rc = NES_EIO;
+        if (rc != NES_OK) {
+          s->last_error = rc;
+          hipStreamSynchronize(sl.stream);
+          finish_batch(s, sl, rc);
+        } else {
+          sl.stage = 2;
+        }
+        progressed = true;
+      } else if (sl.stage == 2 && hipEventQuery(sl.ev_done) == hipSuccess) {
+        finish_batch(s, sl, NES_OK);
+        progressed = true;
+      }
+    }
+    // admit waiting requests into free slots
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      for (auto &slp : s->slots) {
+        Slot &sl = *slp;
+        if (sl.stage != 0 || s->waiting.empty()) continue;
+        form_batch(s, sl);
+        if (sl.reqs.empty()) continue;
+        lk.unlock();
+        int rc = render_setup(sl.work, sl.views.data(), (int)sl.views.size(), sl.stream);
+        if (rc == NES_OK) rc = render_launch_count(scene, sl.work, 0, kMaxSteps, true, sl.stream);
+        if (rc == NES_OK && hipEventRecord(sl.ev_count, sl.stream) != hipSuccess) rc = NES_EIO;
+        sl.launched = Clock::now();
+        if (rc != NES_OK) {
+          s->last_error = rc;
+          hipStreamSynchronize(sl.stream);
+          finish_batch(s, sl, rc);
+        } else {
+          sl.stage = 1;
+        }
+        lk.lock();
+        progressed = true;
+      }
+      bool busy = false;
+      for (auto &slp : s->slots) busy = busy || slp->stage != 0;
+      if (s->stop && !busy && s->waiting.empty()) break;
+      if (!progressed) {
+        if (busy)
+          s->cv_work.wait_for(lk, std::chrono::microseconds(50));
+        else
+          s->cv_work.wait_for(lk, std::chrono::milliseconds(5));
+      }
+    }
+  }
+}
+
+}  // namespace nes
+
+extern "C" {
+
+int nes_server_create(nes_ctx *ctx, const nes_server_config *cfg, nes_server **out) {
+  if (!ctx || !out) return NES_EINVAL;
+  NES_TRY(ensure_device());
+  auto s = std::make_unique<nes_server>();
+  s->ctx = ctx;
+  if (cfg) s->cfg = *cfg;
+  if (!s->cfg.hbm_budget_bytes) s->cfg.hbm_budget_bytes = 48ull << 30;
+  if (!s->cfg.max_batch_rays) s->cfg.max_batch_rays = 1u << 24;
+  if (!s->cfg.max_inflight) s->cfg.max_inflight = 3;
+  if (!s->cfg.queue_capacity) s->cfg.queue_capacity = 256;
+  if (s->cfg.max_batch_rays > kMaxSlots) return NES_ERANGE;
+  if (s->cfg.precision != NES_PRECISION_FP32 && s->cfg.precision != NES_PRECISION_MIXED) return NES_EINVAL;
+  if (s->cfg.sws_mode != NES_SWS_X86 && s->cfg.sws_mode != NES_SWS_C) return NES_EINVAL;
+  // admission by memory: as many batch slots as the budget holds
+  const size_t per_slot = slot_bytes(s->cfg.max_batch_rays, 16);
+  const uint64_t afford = s->cfg.hbm_budget_bytes / per_slot;
+  if (afford < 1) {
+    set_last_error("HBM budget below one batch slot");
+    return NES_ENOMEM;
+  }
+  const uint32_t n_slots = (uint32_t)std::min<uint64_t>(s->cfg.max_inflight, afford);
+  for (uint32_t i = 0; i < n_slots; ++i) {
+    auto sl = std::make_unique<Slot>();
+    NES_HIP_TRY(hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking));
+    NES_HIP_TRY(hipEventCreateWithFlags(&sl->ev_count, hipEventDisableTiming));
+    NES_HIP_TRY(hipEventCreateWithFlags(&sl->ev_done, hipEventDisableTiming));
+    s->slots.push_back(std::move(sl));
+  }
+  nes_server *raw = s.release();
+  raw->scheduler = std::thread(scheduler_loop, raw);
+  *out = raw;
+  return NES_OK;
+}
+
+int nes_server_destroy(nes_server *s) {
+  if (!s) return NES_EINVAL;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+    // requests never admitted are cancelled
+    for (auto &rq : s->waiting) {
+      rq->state = ReqState::kSuperseded;
+      rq->status = NES_ECANCELED_;
+    }
+    s->waiting.clear();
+  }
+  s->cv_work.notify_all();
+  if (s->scheduler.joinable()) s->scheduler.join();
+  for (auto &sl : s->slots) {
+    hipStreamSynchronize(sl->stream);
+    render_work_free(sl->work);
+    sl->yuv.release();
+    hipEventDestroy(sl->ev_count);
+    hipEventDestroy(sl->ev_done);
+    hipStreamDestroy(sl->stream);
+  }
+  for (auto &kv : s->tickets) s->pinned.put(kv.second->yuv);
+  delete s;
+  return NES_OK;
+}
+
+int nes_server_submit(nes_server *s, uint32_t client, const nes_camera *views, int n_views,
+                      uint64_t *ticket) {
+  if (!s || !views || n_views < 1 || n_views > kMaxViews || !ticket) return NES_EINVAL;
+  auto rq = std::make_shared<Request>();
+  rq->client = client;
+  rq->n_views = n_views;
+  size_t total = 0;
+  for (int v = 0; v < n_views; ++v) {
+    const nes_camera &c = views[v];
+    if (c.width < 2 || c.height < 2 || (c.width & 1) || (c.height & 1) || c.width > 16384 ||
+        c.height > 16384 || !(c.fov_y_deg > 0.0f && c.fov_y_deg < 180.0f))
+      return NES_EINVAL;
+    rq->views[v] = c;
+    rq->view_off[v] = total;
+    total += yuv_bytes(c);
+  }
+  NES_TRY(s->pinned.get(total, &rq->yuv));
+  rq->arrival = Clock::now();
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stop) {
+      s->pinned.put(rq->yuv);
+      return NES_EINVAL;
+    }
+    if (s->waiting.size() >= s->cfg.queue_capacity) {
+      s->pinned.put(rq->yuv);
+      return NES_EBUSY;
+    }
+    // latest pose wins: a newer request of the same client replaces one that
+    // has not been admitted yet
+    auto it = s->client_waiting.find(client);
+    if (it != s->client_waiting.end()) {
+      auto old = it->second;
+      old->state = ReqState::kSuperseded;
+      old->status = NES_ECANCELED_;
+      s->waiting.erase(std::remove(s->waiting.begin(), s->waiting.end(), old), s->waiting.end());
+      s->superseded++;
+    }
+    rq->ticket = s->next_ticket++;
+    s->tickets[rq->ticket] = rq;
+    s->client_waiting[client] = rq;
+    s->waiting.push_back(rq);
+    if (!s->any_arrival) {
+      s->any_arrival = true;
+      s->first_arrival = rq->arrival;
+    }
+    *ticket = rq->ticket;
+  }
+  s->cv_work.notify_one();
+  s->cv_done.notify_all();
+  return NES_OK;
+}
+
+int nes_server_wait(nes_server *s, uint64_t ticket, int timeout_ms, nes_frame_result *out) {
+  if (!s || !out) return NES_EINVAL;
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->tickets.find(ticket);
+  if (it == s->tickets.end()) return NES_EINVAL;
+  auto rq = it->second;
+  auto ready = [&] { return rq->state != ReqState::kWaiting && rq->state != ReqState::kRunning; };
+  if (timeout_ms < 0)
+    s->cv_done.wait(lk, ready);
+  else if (!s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready))
+    return NES_EAGAIN;
+  std::memset(out, 0, sizeof(*out));
+  out->ticket = rq->ticket;
+  out->index = rq->views[0].index;
+  out->n_views = (uint32_t)rq->n_views;
+  out->status = rq->status;
+  if (rq->state == ReqState::kDone) {
+    for (int v = 0; v < rq->n_views; ++v) {
+      const nes_camera &c = rq->views[v];
+      out->width[v] = c.width;
+      out->height[v] = c.height;
+      out->y[v] = rq->yuv.ptr + rq->view_off[v];
+      out->u[v] = out->y[v] + (size_t)c.width * c.height;
+      out->v[v] = out->u[v] + (size_t)(c.width / 2) * (c.height / 2);
+    }
+    out->latency_ms = ms_between(rq->arrival, rq->done);
+    out->queue_ms = ms_between(rq->arrival, rq->admitted);
+    out->rays = rq->rays;
+    out->samples = rq->samples;
+  }
+  return rq->status;
+}
+
+int nes_server_release(nes_server *s, uint64_t ticket) {
+  if (!s) return NES_EINVAL;
+  std::shared_ptr<Request> rq;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->tickets.find(ticket);
+    if (it == s->tickets.end()) return NES_EINVAL;
+    rq = it->second;
+    if (rq->state == ReqState::kWaiting || rq->state == ReqState::kRunning) return NES_EBUSY;
+    s->tickets.erase(it);
+  }
+  s->pinned.put(rq->yuv);
+  rq->yuv = PinnedBlock();
+  return NES_OK;
+}
+
+int nes_server_stats_get(nes_server *s, nes_server_stats *st) {
+  if (!s || !st) return NES_EINVAL;
+  std::lock_guard<std::mutex> lk(s->mu);
+  std::memset(st, 0, sizeof(*st));
+  st->completed = s->completed;
+  st->superseded = s->superseded;
+  st->failed = s->failed;
+  st->rays = s->rays;
+  st->samples = s->samples;
+  st->batches = s->batches;
+  st->slots = (uint32_t)s->slots.size();
+  std::vector<double> l = s->latencies_ms;
+  if (!l.empty()) {
+    std::sort(l.begin(), l.end());
+    auto pct = [&](double p) {
+      const double idx = p * (double)(l.size() - 1);
+      const size_t lo = (size_t)std::floor(idx), hi = std::min(lo + 1, l.size() - 1);
+      return l[lo] + (l[hi] - l[lo]) * (idx - (double)lo);
+    };
+    st->p50_ms = pct(0.50);
+    st->p99_ms = pct(0.99);
+    st->max_ms = l.back();
+    double sum = 0;
+    for (double x : l) sum += x;
+    st->mean_ms = sum / (double)l.size();
+  }
+  if (s->any_arrival && s->completed) {
+    st->wall_s = std::chrono::duration<double>(s->last_done - s->first_arrival).count();
+    if (st->wall_s > 0) st->mrays_per_s = (double)s->rays / st->wall_s / 1e6;
+  }
+  return NES_OK;
+}
+
+int nes_server_reset_stats(nes_server *s) {
+  if (!s) return NES_EINVAL;
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->latencies_ms.clear();
+  s->completed = s->superseded = s->failed = s->rays = s->samples = s->batches = 0;
+  s->any_arrival = false;
+  return NES_OK;
+}
+
+// Poisson load generator: n_clients independent clients, each with
+// exponential inter-arrival times at rate_hz, a seeded head pose on a
+// radius-2 orbit, a seeded FOV in [fov_min, fov_max] and a per-eye
+// resolution drawn from res_wh; stereo pairs with 64 mm IPD.  Every frame
+// that completes is waited for and released; returns the server statistics
+// of the run (latency = arrival -> YUV420P planes in host memory).
+int nes_server_simulate(nes_server *s, const nes_sim_config *sc, nes_server_stats *st) {
+  if (!s || !sc || !st || sc->n_clients < 1 || !(sc->rate_hz > 0) || !(sc->duration_s > 0) ||
+      sc->n_res < 1 || !sc->res_wh)
+    return NES_EINVAL;
+  NES_TRY(nes_server_reset_stats(s));
+  struct Arrival {
+    double t;
+    uint32_t client;
+    uint64_t k;
+  };
+  std::vector<Arrival> arr;
+  for (uint32_t c = 0; c < sc->n_clients; ++c) {
+    double t = 0;
+    for (uint64_t k = 0;; ++k) {
+      const double u = rand_uniform(sc->seed, 5000 + c, 4 * k);
+      t += -std::log(1.0 - u) / sc->rate_hz;
+      if (t >= sc->duration_s) break;
+      arr.push_back({t, c, k});
+    }
+  }
+  std::sort(arr.begin(), arr.end(), [](const Arrival &a, const Arrival &b) { return a.t < b.t; });
+  std::mutex tmu;
+  std::condition_variable tcv;
+  std::deque<uint64_t> pending;
+  bool gen_done = false;
+  int first_err = NES_OK;
+  std::thread collector([&] {
+    while (true) {
+      uint64_t t;
+      {
+        std::unique_lock<std::mutex> lk(tmu);
+        tcv.wait(lk, [&] { return !pending.empty() || gen_done; });
+        if (pending.empty()) return;
+        t = pending.front();
+        pending.pop_front();
+      }
+      nes_frame_result r;
+      nes_server_wait(s, t, -1, &r);
+      nes_server_release(s, t);
+    }
+  });
+  const Clock::time_point t0 = Clock::now();
+  for (const Arrival &a : arr) {
+    std::this_thread::sleep_until(t0 + std::chrono::duration_cast<Clock::duration>(
+                                           std::chrono::duration<double>(a.t)));
+    const uint64_t base = 4 * a.k;
+    const double az = 360.0 * rand_uniform(sc->seed, 6000 + a.client, base);
+    const double el = 5.0 + 30.0 * rand_uniform(sc->seed, 6000 + a.client, base + 1);
+    const double fu = rand_uniform(sc->seed, 6000 + a.client, base + 2);
+    const uint32_t ri = (uint32_t)(rand_u64(sc->seed, 6000 + a.client, base + 3) % sc->n_res);
+    nes_camera v[2];
+    for (int e = 0; e < 2; ++e) {
+      std::memset(&v[e], 0, sizeof(nes_camera));
+      v[e].index = a.k;
+      v[e].is_left = e == 0;
+      v[e].width = sc->res_wh[2 * ri];
+      v[e].height = sc->res_wh[2 * ri + 1];
+      v[e].fov_y_deg = (float)(sc->fov_min + (sc->fov_max - sc->fov_min) * fu);
+      nes_orbit_camera(az, el, 2.0, e == 0 ? -0.032 : 0.032, v[e].c2w);
+    }
+    uint64_t ticket = 0;
+    const int rc = nes_server_submit(s, a.client, v, sc->stereo ? 2 : 1, &ticket);
+    if (rc == NES_OK) {
+      std::lock_guard<std::mutex> lk(tmu);
+      pending.push_back(ticket);
+      tcv.notify_one();
+    } else if (rc != NES_EBUSY && first_err == NES_OK) {
+      first_err = rc;
+    }
+  }
+  {
+    std::lock_guard<std::mutex> lk(tmu);
+    gen_done = true;
+  }
+  tcv.notify_all();
+  collector.join();
+  NES_TRY(first_err);
+  return nes_server_stats_get(s, st);
+}
+
+}  // extern "C"

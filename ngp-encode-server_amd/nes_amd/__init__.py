@@ -59,6 +59,38 @@ class RenderStats(C.Structure):
                 ("ms_eval", C.c_double), ("ms_march", C.c_double), ("ms_composite", C.c_double)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_batch_rays", C.c_uint32),
+                ("max_inflight", C.c_uint32), ("queue_capacity", C.c_uint32),
+                ("precision", C.c_int), ("sws_mode", C.c_int)]
+
+
+class FrameResult(C.Structure):
+    _fields_ = [("ticket", C.c_uint64), ("index", C.c_uint64), ("n_views", C.c_uint32),
+                ("width", C.c_uint32 * 2), ("height", C.c_uint32 * 2),
+                ("y", C.POINTER(C.c_uint8) * 2), ("u", C.POINTER(C.c_uint8) * 2),
+                ("v", C.POINTER(C.c_uint8) * 2), ("latency_ms", C.c_double),
+                ("queue_ms", C.c_double), ("rays", C.c_uint64), ("samples", C.c_uint64),
+                ("status", C.c_int)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("completed", C.c_uint64), ("superseded", C.c_uint64), ("failed", C.c_uint64),
+                ("rays", C.c_uint64), ("samples", C.c_uint64), ("batches", C.c_uint64),
+                ("slots", C.c_uint32), ("p50_ms", C.c_double), ("p99_ms", C.c_double),
+                ("mean_ms", C.c_double), ("max_ms", C.c_double), ("wall_s", C.c_double),
+                ("mrays_per_s", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class SimConfig(C.Structure):
+    _fields_ = [("n_clients", C.c_uint32), ("rate_hz", C.c_double), ("duration_s", C.c_double),
+                ("seed", C.c_uint64), ("n_res", C.c_uint32), ("res_wh", C.POINTER(C.c_uint32)),
+                ("fov_min", C.c_double), ("fov_max", C.c_double), ("stereo", C.c_int)]
+
+
 # C-ABI symbols declared in include/nes_render.h (checked by the CPU tests).
 _SIGS = {}
 
@@ -91,6 +123,14 @@ _sig("nes_gray8_to_yuv420p", C.c_int, _P(C.c_uint8), C.c_int, C.c_int, C.c_int, 
      C.c_int, _P(C.c_uint8), C.c_int, _P(C.c_uint8), C.c_int, C.c_int)
 _sig("nes_sws_vchroma_filter", C.c_int, C.c_int, _P(C.c_int16), _P(C.c_int32), C.c_int,
      _P(C.c_int))
+_sig("nes_server_create", C.c_int, C.c_void_p, _P(ServerConfig), _P(C.c_void_p))
+_sig("nes_server_destroy", C.c_int, C.c_void_p)
+_sig("nes_server_submit", C.c_int, C.c_void_p, C.c_uint32, _P(Camera), C.c_int, _P(C.c_uint64))
+_sig("nes_server_wait", C.c_int, C.c_void_p, C.c_uint64, C.c_int, _P(FrameResult))
+_sig("nes_server_release", C.c_int, C.c_void_p, C.c_uint64)
+_sig("nes_server_stats_get", C.c_int, C.c_void_p, _P(ServerStats))
+_sig("nes_server_reset_stats", C.c_int, C.c_void_p)
+_sig("nes_server_simulate", C.c_int, C.c_void_p, _P(SimConfig), _P(ServerStats))
 _sig("nes_device_alloc", C.c_int, C.c_size_t, _P(C.c_void_p))
 _sig("nes_device_free", C.c_int, C.c_void_p)
 _sig("nes_memcpy_h2d", C.c_int, C.c_void_p, C.c_void_p, C.c_size_t)
@@ -310,3 +350,75 @@ def device_info():
     hbm = C.c_uint64()
     check(lib().nes_device_info(name, 64, C.byref(cus), C.byref(hbm)), "nes_device_info")
     return name.value.decode(), cus.value, hbm.value
+
+
+# --------------------------------------------------------------- serving
+
+NES_ECANCELED = -125
+
+
+class Server:
+    """Continuous-batching render server over a Context (nes_server)."""
+
+    def __init__(self, ctx, hbm_budget=0, max_batch_rays=0, max_inflight=0, queue_capacity=0,
+                 precision=PRECISION_MIXED, sws_mode=SWS_X86):
+        self.ctx = ctx
+        cfg = ServerConfig(hbm_budget, max_batch_rays, max_inflight, queue_capacity, precision,
+                           sws_mode)
+        h = C.c_void_p()
+        check(lib().nes_server_create(ctx.handle, C.byref(cfg), C.byref(h)), "nes_server_create")
+        self.handle = h
+
+    def submit(self, client, views):
+        if isinstance(views, Camera):
+            views = [views]
+        arr = (Camera * len(views))(*views)
+        t = C.c_uint64()
+        check(lib().nes_server_submit(self.handle, client, arr, len(views), C.byref(t)),
+              "nes_server_submit")
+        return t.value
+
+    def wait(self, ticket, timeout_ms=-1, release=True):
+        """Returns (status, [(y, u, v) per view] copied out, FrameResult)."""
+        r = FrameResult()
+        rc = lib().nes_server_wait(self.handle, ticket, timeout_ms, C.byref(r))
+        planes = []
+        if rc == NES_OK:
+            for i in range(r.n_views):
+                w, h = r.width[i], r.height[i]
+                y = np.ctypeslib.as_array(r.y[i], (h, w)).copy()
+                u = np.ctypeslib.as_array(r.u[i], (h // 2, w // 2)).copy()
+                v = np.ctypeslib.as_array(r.v[i], (h // 2, w // 2)).copy()
+                planes.append((y, u, v))
+        elif rc not in (NES_ECANCELED, NES_EAGAIN):
+            check(rc, "nes_server_wait")
+        if release and rc != NES_EAGAIN:
+            check(lib().nes_server_release(self.handle, ticket), "nes_server_release")
+        return rc, planes, r
+
+    def stats(self):
+        st = ServerStats()
+        check(lib().nes_server_stats_get(self.handle, C.byref(st)), "nes_server_stats_get")
+        return st
+
+    def simulate(self, n_clients=8, rate_hz=72.0, duration_s=2.0, seed=4,
+                 resolutions=((640, 480), (1440, 1600), (2064, 2208)), fov=(60.0, 90.0),
+                 stereo=True):
+        res = np.ascontiguousarray(np.array(resolutions, np.uint32).reshape(-1))
+        sc = SimConfig(n_clients, rate_hz, duration_s, seed, len(resolutions),
+                       res.ctypes.data_as(C.POINTER(C.c_uint32)), fov[0], fov[1], int(stereo))
+        st = ServerStats()
+        check(lib().nes_server_simulate(self.handle, C.byref(sc), C.byref(st)),
+              "nes_server_simulate")
+        return st
+
+    def close(self):
+        if self.handle:
+            check(lib().nes_server_destroy(self.handle), "nes_server_destroy")
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
