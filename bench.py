@@ -44,27 +44,26 @@ def time_render(N, ctx, views, precision, steps, warmup):
 
 
 def cpu_baseline(N, table, weights, occ, ctx, views, budget_s):
-    """FP64 oracle on a bounded, evenly strided subset of the same rays."""
+    """FP64 oracle on the same rays (whole frames, or an even stride of them
+    when that would exceed about budget_s seconds); its pixels double as the
+    PSNR reference of the mixed-precision frames."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     threads = min(16, os.cpu_count() or 1)
     outs, _ = ctx.render(views, precision=N.PRECISION_MIXED)
     w, h = views[0].width, views[0].height
-    stride = 9
+    ocams = [O.camera(w, h, list(v.c2w), fov=v.fov_y_deg) for v in views]
+    probe = np.arange(0, w * h, 97, dtype=np.uint32)
+    t0 = time.perf_counter()
+    O.render_pixels(table, weights, occ, ocams[0], probe, threads=threads)
+    per_ray = (time.perf_counter() - t0) / len(probe)
+    stride = max(1, int(math.ceil(per_ray * w * h * len(views) / budget_s)))
     total_rays, total_t = 0, 0.0
     ref, got = [], []
-    for v, o in zip(views, outs):
-        ocam = O.camera(w, h, list(v.c2w), fov=v.fov_y_deg)
+    for oc, o in zip(ocams, outs):
         pix = np.arange(0, w * h, stride, dtype=np.uint32)
-        # time a slice first to bound the run to about budget_s seconds
-        probe = pix[:4096]
         t0 = time.perf_counter()
-        O.render_pixels(table, weights, occ, ocam, probe, threads=threads)
-        dt = time.perf_counter() - t0
-        keep = int(min(len(pix), max(4096, budget_s / 2 / max(dt, 1e-9) * 4096)))
-        pix = pix[:: max(1, len(pix) // keep)]
-        t0 = time.perf_counter()
-        rgb, depth, ns = O.render_pixels(table, weights, occ, ocam, pix, threads=threads)
+        rgb, depth, ns = O.render_pixels(table, weights, occ, oc, pix, threads=threads)
         total_t += time.perf_counter() - t0
         total_rays += len(pix)
         ref.append(rgb)
@@ -72,11 +71,21 @@ def cpu_baseline(N, table, weights, occ, ctx, views, budget_s):
     ref, got = np.concatenate(ref), np.concatenate(got)
     mse = float(np.mean((ref - got) ** 2))
     psnr = 10 * math.log10(1.0 / mse) if mse > 0 else float("inf")
+    what = "both full 1920x1080 eyes" if stride == 1 else f"every {stride}th pixel of both eyes"
     base = {"value": total_rays / total_t / 1e6, "unit": "Mrays/s", "cores": threads,
             "kind": "port",
-            "sample": f"FP64 oracle render of {total_rays} rays (every {stride}th-or-sparser "
-                      f"pixel of both 1920x1080 eyes), {total_t:.1f} s"}
+            "sample": f"FP64 oracle render of {total_rays} rays ({what}), {total_t:.1f} s"}
     return base, psnr, total_rays
+
+
+def serving_leg(N, ctx, seconds):
+    """8 Poisson clients at 72 Hz, stereo, per-eye {640x480, 1440x1600, 2064x2208}."""
+    with N.Server(ctx) as srv:
+        srv.simulate(n_clients=8, rate_hz=72.0, duration_s=0.3, seed=99)  # warm-up
+        st = srv.simulate(n_clients=8, rate_hz=72.0, duration_s=seconds, seed=4)
+    d = st.as_dict()
+    d["offered_requests_per_s"] = 8 * 72.0
+    return d
 
 
 def frameprep_leg(N, n_frames=256, w=1280, h=720):
@@ -119,6 +128,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-frameprep", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=3.0)
+    ap.add_argument("--skip-serve", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -168,6 +179,10 @@ def main():
         }
         if not args.skip_frameprep:
             result["frameprep"] = frameprep_leg(N)
+        if not args.skip_serve:
+            srv = serving_leg(N, ctx, args.serve_seconds)
+            result["serving"] = srv
+            result["p99_frame_latency_ms"] = round(srv["p99_ms"], 3)
         if not args.skip_cpu:
             base, psnr, nr = cpu_baseline(N, table, weights, occ, ctx, views, args.cpu_seconds)
             result["cpu_baseline"] = base

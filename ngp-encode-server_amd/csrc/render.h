@@ -34,10 +34,12 @@ struct RayAcc {  // 32 bytes, compositing state carried across passes
 constexpr uint32_t kMaxSlots = 1u << 25;
 
 struct RenderWork {
-  DevBuf views, rays, acc, counts, offsets, samples, sample_out, out, total;
+  DevBuf views, rays, acc, counts, offsets, samples, sample_out, sample_pos, feat, out, total;
   uint32_t *total_host = nullptr;  // pinned
   hipEvent_t ev[8] = {};
   uint32_t n_views = 0, n_slots = 0;
+  uint32_t capacity = 0;          // samples the sample buffers hold
+  int capacity_precision = -1;    // feature layout the buffers were sized for
   size_t out_px = 0;
   std::vector<ViewDesc> host_views;
 };
@@ -62,16 +64,19 @@ struct OutLayout {
 };
 OutLayout out_layout(size_t px);
 
-// Stages of one render (single pass: k0 = 0, k1 = 128, first = last = true):
-//   render_setup        views -> ray-slot layout, buffers, view upload
-//   render_launch_count march_count + async read-back of the sample total
-//                       into work.total_host[0]
-//   render_launch_rest  (after the total is known) march_fill, eval, composite
+// Stages of one render:
+//   render_setup            views -> ray-slot layout, ray buffers, view upload
+//   render_prepare_capacity sample buffers for the expected sample count
+//   render_launch_pass      march_emit, encode, MLP, composite for march steps
+//                           [k0, k1), then an async copy of the pass's sample
+//                           total into work.total_host[0].  A total above
+//                           work.capacity means the pass was dropped: grow with
+//                           render_reserve_samples and launch the pass again.
 int render_setup(RenderWork &w, const nes_camera *views, int n_views, hipStream_t stream);
-int render_launch_count(const SceneDev &scene, RenderWork &w, int k0, int k1, bool first,
-                        hipStream_t stream);
-int render_launch_rest(const SceneDev &scene, RenderWork &w, int precision, int k0, int k1,
-                       bool last, uint32_t n, hipStream_t stream, hipEvent_t ev_eval_begin,
+int render_prepare_capacity(RenderWork &w, int precision);
+int render_reserve_samples(RenderWork &w, int precision, size_t capacity);
+int render_launch_pass(const SceneDev &scene, RenderWork &w, int precision, int k0, int k1,
+                       bool first, bool last, hipStream_t stream, hipEvent_t ev_eval_begin,
                        hipEvent_t ev_eval_end);
 
 // Enqueue a full render of views on stream; returns after the sample count

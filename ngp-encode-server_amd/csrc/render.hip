@@ -5,17 +5,20 @@
 //
 // Pipeline per pass (a pass covers a window of march steps; one pass of 128
 // steps is the default):
-//   march_count  one thread per ray slot: camera ray, box clip, count
-//                occupied steps; wave-aggregated atomic for the pass total
-//   march_fill   same walk, writes 4-byte sample records (slot << 7 | step)
-//                into per-ray contiguous segments (wave scan + one atomic)
-//   eval_*       the network over the dense sample list:
-//                  eval_fp32  FP32 table, per-lane FP32 VALU MLP
-//                  eval_mixed FP16 table, FP16 MFMA MLP with FP32 accumulate
-//   composite    one thread per ray: front-to-back alpha compositing in FP32,
-//                stops at T < 1e-4, writes RGB24 / depth / float RGB
-// Both precisions share every kernel except eval_*, so their throughput ratio
-// isolates the encoding + MLP arithmetic.
+//   march_emit    one thread per ray slot: camera ray, box clip, occupied
+//                 steps; a wave scan + one atomic per wavefront reserves a
+//                 contiguous segment of the sample list on the device
+//   encode_parts  16-level hash-grid encoding, XCD-partitioned: workgroups on
+//                 XCD p encode levels p and 15-p, so each XCD's L2 caches two
+//                 levels of the table
+//   mlp_*         density + colour MLPs over the dense sample list:
+//                   mlp_fp32  FP32 table features, per-lane FP32 VALU MLP
+//                   mlp_mixed FP16 table features, FP16 MFMA MLP, FP32 accum
+//   composite     one thread per ray: front-to-back alpha compositing in FP32,
+//                 stops at T < 1e-4, writes RGB24 / depth / float RGB
+// Both precisions share every kernel except the table type in encode_parts
+// and the MLP kernel, so their throughput ratio isolates those.  The sample
+// total stays on the device (no host round trip inside a render).
 //
 // Sample geometry (ray, box clip, step positions, occupancy cell) is defined
 // in FP32 with explicit fused multiply-adds and correctly rounded division and
@@ -120,16 +123,25 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
   return v;
 }
 
-// ----------------------------------------------------------- march kernels
-__global__ __launch_bounds__(256) void march_count(
+// ----------------------------------------------------------- march kernel
+// One kernel for both march walks: count this ray's occupied steps, reserve
+// a contiguous segment of the sample list (wave scan + one atomic per
+// wavefront on a device-side cursor), then write the records.  The sample
+// total never leaves the device: later kernels read it from the cursor, and
+// writes beyond `capacity` are dropped (the host re-renders with a larger
+// buffer when the final total exceeds it).
+__global__ __launch_bounds__(256) void march_emit(
     const ViewDesc *__restrict__ views, uint32_t n_views, uint32_t n_slots,
-    const uint32_t *__restrict__ occ, RayData *rays, RayAcc *acc,
-    uint32_t *counts, uint32_t *total, int k_begin, int k_end, int first) {
+    const uint32_t *__restrict__ occ, RayData *rays, RayAcc *acc, uint32_t *counts,
+    uint32_t *offsets, uint32_t *samples, float4 *__restrict__ pos, uint32_t *cursor,
+    uint32_t capacity, int k_begin, int k_end, int first) {
   const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = lane_id();
   uint32_t cnt = 0;
+  uint32_t bits[4] = {0u, 0u, 0u, 0u};  // occupied steps of this pass
+  RayData r;
+  bool active = false;
   if (slot < n_slots) {
-    RayData r;
-    bool active;
     if (first) {
       uint32_t vi = 0;
       for (uint32_t i = 1; i < n_views; ++i)
@@ -165,23 +177,13 @@ __global__ __launch_bounds__(256) void march_count(
       for (int k = k_begin; k < kend; ++k) {
         float x[3], t;
         step_pos(r, k, x, &t);
-        cnt += occupied(occ, x) ? 1u : 0u;
+        if (occupied(occ, x)) {
+          bits[k >> 5] |= 1u << (k & 31);
+          ++cnt;
+        }
       }
     }
-    counts[slot] = cnt;
   }
-  const uint32_t s = wave_sum(cnt);
-  if (lane_id() == 0 && s) atomicAdd(total, s);
-}
-
-__global__ __launch_bounds__(256) void march_fill(
-    uint32_t n_slots, const uint32_t *__restrict__ occ,
-    const RayData *__restrict__ rays, const uint32_t *__restrict__ counts,
-    uint32_t *offsets, uint32_t *samples, uint32_t *cursor, int k_begin,
-    int k_end) {
-  const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
-  const int lane = lane_id();
-  const uint32_t cnt = slot < n_slots ? counts[slot] : 0u;
   uint32_t incl = cnt;
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
@@ -194,78 +196,27 @@ __global__ __launch_bounds__(256) void march_fill(
   base = __shfl(base, 0);
   if (slot >= n_slots) return;
   uint32_t off = base + incl - cnt;
+  counts[slot] = cnt;
   offsets[slot] = off;
-  if (!cnt) return;
-  const RayData r = load_ray(rays, slot);
-  const int kend = min(k_end, (int)r.n_steps);
-  for (int k = k_begin; k < kend; ++k) {
-    float x[3], t;
-    step_pos(r, k, x, &t);
-    if (occupied(occ, x)) samples[off++] = (slot << 7) | (uint32_t)k;
+#pragma unroll
+  for (int wd = 0; wd < 4; ++wd) {
+    uint32_t b = bits[wd];
+    while (b) {
+      const int k = 32 * wd + __builtin_ctz(b);
+      b &= b - 1;
+      if (off < capacity) {
+        float x[3], t;
+        step_pos(r, k, x, &t);
+        pos[off] = make_float4(fminf(fmaxf(x[0], 0.0f), 1.0f), fminf(fmaxf(x[1], 0.0f), 1.0f),
+                               fminf(fmaxf(x[2], 0.0f), 1.0f), __uint_as_float(slot));
+        samples[off] = (slot << 7) | (uint32_t)k;
+      }
+      ++off;
+    }
   }
 }
 
 // --------------------------------------------------------------- encoding
-template <int L>
-__device__ __forceinline__ uint32_t level_entry(uint32_t x, uint32_t y, uint32_t z) {
-  if constexpr (L < kDenseLevels) {
-    constexpr uint32_t n1 = (uint32_t)kLevelRes[L] + 1u;
-    return kLevelOffset[L] + x + y * n1 + z * (n1 * n1);
-  } else {
-    return kLevelOffset[L] + ((x ^ (y * kPrime1) ^ (z * kPrime2)) & ((1u << kLog2T) - 1u));
-  }
-}
-
-__device__ __forceinline__ float2 load_feat(const float *__restrict__ t, uint32_t e) {
-  return __ldg(reinterpret_cast<const float2 *>(t) + e);
-}
-__device__ __forceinline__ float2 load_feat(const _Float16 *__restrict__ t, uint32_t e) {
-  const uint32_t bits = __ldg(reinterpret_cast<const uint32_t *>(t) + e);
-  half2v h = __builtin_bit_cast(half2v, bits);
-  return make_float2((float)h.x, (float)h.y);
-}
-
-template <int L, class Tab>
-__device__ __forceinline__ void encode_level(const Tab *__restrict__ table,
-                                             const float xc[3], float &f0,
-                                             float &f1) {
-  constexpr float scale = (float)kLevelRes[L];
-  uint32_t i0[3];
-  float fr[3];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    const float p = xc[a] * scale;
-    const float fl = fminf(floorf(p), scale - 1.0f);
-    i0[a] = (uint32_t)fl;
-    fr[a] = p - fl;
-  }
-  float2 v[8];
-#pragma unroll
-  for (int c = 0; c < 8; ++c)
-    v[c] = load_feat(table, level_entry<L>(i0[0] + (c & 1), i0[1] + ((c >> 1) & 1),
-                                           i0[2] + ((c >> 2) & 1)));
-  const float gx = 1.0f - fr[0], gy = 1.0f - fr[1], gz = 1.0f - fr[2];
-  float a0 = 0.0f, a1 = 0.0f;
-#pragma unroll
-  for (int c = 0; c < 8; ++c) {
-    const float w = ((c & 1) ? fr[0] : gx) * (((c >> 1) & 1) ? fr[1] : gy) *
-                    (((c >> 2) & 1) ? fr[2] : gz);
-    a0 = fmaf(w, v[c].x, a0);
-    a1 = fmaf(w, v[c].y, a1);
-  }
-  f0 = a0;
-  f1 = a1;
-}
-
-template <class Tab, int L = 0>
-__device__ __forceinline__ void encode(const Tab *__restrict__ table,
-                                       const float xc[3], float feat[32]) {
-  if constexpr (L < kLevels) {
-    encode_level<L>(table, xc, feat[2 * L], feat[2 * L + 1]);
-    encode<Tab, L + 1>(table, xc, feat);
-  }
-}
-
 __device__ __forceinline__ void sh4(const float d[3], float sh[16]) {
   const float x = d[0], y = d[1], z = d[2];
   const float xx = x * x, yy = y * y, zz = z * z;
@@ -287,65 +238,6 @@ __device__ __forceinline__ void sh4(const float d[3], float sh[16]) {
   sh[15] = 0.59004358992664352f * x * (-xx + 3.0f * yy);
 }
 
-__device__ __forceinline__ void sample_geometry(const uint32_t *__restrict__ samples,
-                                                const RayData *__restrict__ rays,
-                                                uint32_t i, float xc[3], float d[3]) {
-  const uint32_t rec = samples[i];
-  const RayData r = load_ray(rays, rec >> 7);
-  float x[3], t;
-  step_pos(r, (int)(rec & 127u), x, &t);
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    xc[a] = fminf(fmaxf(x[a], 0.0f), 1.0f);
-    d[a] = r.d[a];
-  }
-}
-
-// ---------------------------------------------------- level-split encode
-// A wavefront encodes 16 samples at a time: lane (g = lane / 16, j = lane % 16)
-// computes levels 4g .. 4g+3 of sample j, i.e. 32 independent corner gathers
-// per lane.  Each lane always serves the same four levels, so their constants
-// live in registers and every lane runs the same instruction stream (dense and
-// hashed indices are both formed and selected per lane).  The eight features
-// a lane produces are exactly the K-slots (g, 0..7) of the first MFMA's B
-// operand for sample j.
-struct LevelParams {
-  float scale[4];
-  uint32_t n1[4], n1sq[4], off[4];
-  bool dense[4];
-};
-
-__device__ __forceinline__ LevelParams level_params(int g) {
-  LevelParams p;
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const int l = 4 * g + q;
-    // kLevelRes / kLevelOffset are constexpr; a 4-way select keeps them out
-    // of memory
-    int res = 0;
-    uint32_t off = 0;
-#pragma unroll
-    for (int c = 0; c < 4; ++c)
-      if (g == c) {
-        res = kLevelRes[4 * c + q];
-        off = kLevelOffset[4 * c + q];
-      }
-    p.scale[q] = (float)res;
-    p.n1[q] = (uint32_t)res + 1u;
-    p.n1sq[q] = p.n1[q] * p.n1[q];
-    p.off[q] = off;
-    p.dense[q] = l < kDenseLevels;
-  }
-  return p;
-}
-
-__device__ __forceinline__ uint32_t corner_entry(const LevelParams &p, int q, uint32_t x,
-                                                 uint32_t y, uint32_t z) {
-  const uint32_t dense = x + y * p.n1[q] + z * p.n1sq[q];
-  const uint32_t hash = (x ^ (y * kPrime1) ^ (z * kPrime2)) & ((1u << kLog2T) - 1u);
-  return p.off[q] + (p.dense[q] ? dense : hash);
-}
-
 template <class Tab> struct RawFeat;
 template <> struct RawFeat<float> {
   typedef float2 type;
@@ -364,49 +256,6 @@ template <> struct RawFeat<_Float16> {
     return make_float2((float)h.x, (float)h.y);
   }
 };
-
-// Eight features (levels 4g..4g+3, two each) of one sample, FP32.
-template <class Tab>
-__device__ __forceinline__ void encode4(const Tab *__restrict__ table, const LevelParams &lp,
-                                        const float xc[3], float f[8]) {
-  uint32_t e[4][8];
-  float fr[4][3];
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    uint32_t i0[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      const float pp = xc[a] * lp.scale[q];
-      const float fl = fminf(floorf(pp), lp.scale[q] - 1.0f);
-      i0[a] = (uint32_t)fl;
-      fr[q][a] = pp - fl;
-    }
-#pragma unroll
-    for (int c = 0; c < 8; ++c)
-      e[q][c] = corner_entry(lp, q, i0[0] + (c & 1), i0[1] + ((c >> 1) & 1), i0[2] + ((c >> 2) & 1));
-  }
-  // raw words stay packed until they are consumed (1 VGPR per FP16 corner)
-  typename RawFeat<Tab>::type v[4][8];
-#pragma unroll
-  for (int q = 0; q < 4; ++q)
-#pragma unroll
-    for (int c = 0; c < 8; ++c) v[q][c] = RawFeat<Tab>::load(table, e[q][c]);
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const float gx = 1.0f - fr[q][0], gy = 1.0f - fr[q][1], gz = 1.0f - fr[q][2];
-    float a0 = 0.0f, a1 = 0.0f;
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-      const float w = ((c & 1) ? fr[q][0] : gx) * (((c >> 1) & 1) ? fr[q][1] : gy) *
-                      (((c >> 2) & 1) ? fr[q][2] : gz);
-      const float2 fv = RawFeat<Tab>::unpack(v[q][c]);
-      a0 = fmaf(w, fv.x, a0);
-      a1 = fmaf(w, fv.y, a1);
-    }
-    f[2 * q] = a0;
-    f[2 * q + 1] = a1;
-  }
-}
 
 __device__ __forceinline__ float sigmoidf_(float v) { return 1.0f / (1.0f + expf(-v)); }
 
@@ -432,58 +281,7 @@ __device__ __forceinline__ void dense_f32(const float *w, const float *in, float
   }
 }
 
-// 64 samples per wavefront iteration: four level-split encode passes of 16
-// samples fill an LDS feature block, then each lane runs one sample's MLPs in
-// FP32 with the 9344 weights read from LDS as wave-uniform broadcasts.
 constexpr int kEvalWaves = 4;
-
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void eval_fp32(
-    const uint32_t *__restrict__ samples, uint32_t n,
-    const RayData *__restrict__ rays, const float *__restrict__ table,
-    const float *__restrict__ Wg, float4 *__restrict__ out) {
-  __shared__ __attribute__((aligned(16))) float W[kWeights];
-  __shared__ __attribute__((aligned(16))) float feat_lds[kEvalWaves][64][33];
-  for (int i = (int)threadIdx.x; i < kWeights / 4; i += (int)blockDim.x)
-    reinterpret_cast<float4 *>(W)[i] = reinterpret_cast<const float4 *>(Wg)[i];
-  __syncthreads();
-  const int lane = lane_id(), wave = (int)(threadIdx.x >> 6);
-  const int j = lane & 15, g = lane >> 4;
-  const LevelParams lp = level_params(g);
-  const uint32_t n_chunks = (n + 63) / 64;
-  for (uint32_t chunk = blockIdx.x * kEvalWaves + wave; chunk < n_chunks;
-       chunk += gridDim.x * kEvalWaves) {
-#pragma unroll 1
-    for (int t = 0; t < 4; ++t) {
-      const uint32_t s = min(chunk * 64 + 16 * t + j, n - 1);
-      float xc[3], d[3], f[8];
-      sample_geometry(samples, rays, s, xc, d);
-      encode4(table, lp, xc, f);
-#pragma unroll
-      for (int e = 0; e < 8; ++e) feat_lds[wave][16 * t + j][8 * g + e] = f[e];
-    }
-    wave_lds_fence();
-    const uint32_t i = chunk * 64 + (uint32_t)lane;
-    if (i < n) {
-      float xc[3], d[3], feat[32];
-      sample_geometry(samples, rays, i, xc, d);
-#pragma unroll
-      for (int e = 0; e < 32; ++e) feat[e] = feat_lds[wave][lane][e];
-      float h1[64], o2[16], in3[31], h3[64], h4[64], o5[3], sh[16];
-      dense_f32<32, 64, true>(W + kW1, feat, h1);
-      dense_f32<64, 16, false>(W + kW2, h1, o2);
-      sh4(d, sh);
-#pragma unroll
-      for (int k = 0; k < 16; ++k) in3[k] = sh[k];
-#pragma unroll
-      for (int k = 0; k < 15; ++k) in3[16 + k] = o2[1 + k];
-      dense_f32<31, 64, true>(W + kW3, in3, h3);
-      dense_f32<64, 64, true>(W + kW4, h3, h4);
-      dense_f32<64, 3, false>(W + kW5, h4, o5);
-      out[i] = make_float4(expf(o2[0]), sigmoidf_(o5[0]), sigmoidf_(o5[1]), sigmoidf_(o5[2]));
-    }
-    wave_lds_fence();
-  }
-}
 
 // ----------------------------------------------------------- mixed eval
 // MFMA tile: v_mfma_f32_16x16x32_f16, D[16x16] += A[16x32] B[32x16].
@@ -523,29 +321,188 @@ __device__ __forceinline__ half4 sh4_quarter(const float d[3], int g) {
   return r;
 }
 
-__global__ __launch_bounds__(256) void eval_mixed(
-    const uint32_t *__restrict__ samples, uint32_t n,
-    const RayData *__restrict__ rays, const _Float16 *__restrict__ table,
-    const half8 *__restrict__ wfrag, float4 *__restrict__ out) {
+// ------------------------------------------------- XCD-partitioned encode
+// MI355X dispatches workgroup b to XCD b % 8, and each XCD has its own 4 MB
+// L2.  Part p = b % 8 encodes only levels p and 15 - p for every sample, so
+// an XCD's L2 holds one coarse level plus one (2 MB in FP16) hashed level and
+// the corner gathers hit in L2 instead of streaming 64-byte lines from the
+// Infinity Cache.  (In FP32 a hashed level alone is 4 MB.)  Features go to
+// feat[p][sample] as {level p f0, f1, level 15-p f0, f1}.
+constexpr int kParts = 8;
+
+struct PartLevels {
+  float scale[2];
+  uint32_t n1[2], n1sq[2], off[2];
+  bool dense[2];
+};
+
+__device__ __forceinline__ PartLevels part_levels(int part) {
+  PartLevels p;
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int l = q == 0 ? part : 15 - part;
+    int res = 0;
+    uint32_t off = 0;
+#pragma unroll
+    for (int c = 0; c < 16; ++c)
+      if (l == c) {
+        res = kLevelRes[c];
+        off = kLevelOffset[c];
+      }
+    p.scale[q] = (float)res;
+    p.n1[q] = (uint32_t)res + 1u;
+    p.n1sq[q] = p.n1[q] * p.n1[q];
+    p.off[q] = off;
+    p.dense[q] = l < kDenseLevels;
+  }
+  return p;
+}
+
+template <class Tab>
+__device__ __forceinline__ float2 encode_one(const Tab *__restrict__ table, const PartLevels &lp,
+                                             int q, const float xc[3]) {
+  uint32_t i0[3];
+  float fr[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float pp = xc[a] * lp.scale[q];
+    const float fl = fminf(floorf(pp), lp.scale[q] - 1.0f);
+    i0[a] = (uint32_t)fl;
+    fr[a] = pp - fl;
+  }
+  typename RawFeat<Tab>::type v[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const uint32_t x = i0[0] + (c & 1), y = i0[1] + ((c >> 1) & 1), z = i0[2] + ((c >> 2) & 1);
+    const uint32_t dense = x + y * lp.n1[q] + z * lp.n1sq[q];
+    const uint32_t hash = (x ^ (y * kPrime1) ^ (z * kPrime2)) & ((1u << kLog2T) - 1u);
+    v[c] = RawFeat<Tab>::load(table, lp.off[q] + (lp.dense[q] ? dense : hash));
+  }
+  const float gx = 1.0f - fr[0], gy = 1.0f - fr[1], gz = 1.0f - fr[2];
+  float a0 = 0.0f, a1 = 0.0f;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const float w = ((c & 1) ? fr[0] : gx) * (((c >> 1) & 1) ? fr[1] : gy) *
+                    (((c >> 2) & 1) ? fr[2] : gz);
+    const float2 fv = RawFeat<Tab>::unpack(v[c]);
+    a0 = fmaf(w, fv.x, a0);
+    a1 = fmaf(w, fv.y, a1);
+  }
+  return make_float2(a0, a1);
+}
+
+template <class Tab> struct PartOut;
+template <> struct PartOut<float> {
+  typedef float4 type;
+  static __device__ __forceinline__ float4 pack(float2 a, float2 b) { return make_float4(a.x, a.y, b.x, b.y); }
+};
+template <> struct PartOut<_Float16> {
+  typedef uint2 type;
+  static __device__ __forceinline__ uint2 pack(float2 a, float2 b) {
+    half2v ha, hb;
+    ha.x = (_Float16)a.x; ha.y = (_Float16)a.y;
+    hb.x = (_Float16)b.x; hb.y = (_Float16)b.y;
+    return make_uint2(__builtin_bit_cast(uint32_t, ha), __builtin_bit_cast(uint32_t, hb));
+  }
+};
+
+template <class Tab>
+__global__ __launch_bounds__(256) void encode_parts(const float4 *__restrict__ pos,
+                                                    const uint32_t *__restrict__ n_ptr,
+                                                    uint32_t stride, const Tab *__restrict__ table,
+                                                    typename PartOut<Tab>::type *__restrict__ feat,
+                                                    uint32_t blocks_per_part) {
+  const uint32_t n = min(*n_ptr, stride);
+  const int part = (int)(blockIdx.x % kParts);
+  const uint32_t grp = blockIdx.x / kParts;
+  const PartLevels lp = part_levels(part);
+  typename PartOut<Tab>::type *dst = feat + (size_t)part * stride;
+  for (uint32_t s = grp * blockDim.x + threadIdx.x; s < n; s += blocks_per_part * blockDim.x) {
+    const float4 p = pos[s];
+    const float xc[3] = {p.x, p.y, p.z};
+    const float2 fa = encode_one(table, lp, 0, xc);
+    const float2 fb = encode_one(table, lp, 1, xc);
+    dst[s] = PartOut<Tab>::pack(fa, fb);
+  }
+}
+
+// Feature index of part p, component c (the order encode_parts writes).
+__host__ __device__ constexpr int part_feature(int p, int c) {
+  return c < 2 ? 2 * p + c : 2 * (15 - p) + (c - 2);
+}
+
+// FP32 MLPs, one lane per sample, weights in LDS as wave-uniform broadcasts.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void mlp_fp32(
+    const float4 *__restrict__ pos, const float4 *__restrict__ feat,
+    const uint32_t *__restrict__ n_ptr, uint32_t stride, const RayData *__restrict__ rays,
+    const float *__restrict__ Wg, float4 *__restrict__ out) {
+  const uint32_t n = min(*n_ptr, stride);
+  __shared__ __attribute__((aligned(16))) float W[kWeights];
+  for (int i = (int)threadIdx.x; i < kWeights / 4; i += (int)blockDim.x)
+    reinterpret_cast<float4 *>(W)[i] = reinterpret_cast<const float4 *>(Wg)[i];
+  __syncthreads();
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += gridDim.x * blockDim.x) {
+    float f[32];
+#pragma unroll
+    for (int p = 0; p < kParts; ++p) {
+      const float4 v = feat[(size_t)p * stride + i];
+      f[part_feature(p, 0)] = v.x;
+      f[part_feature(p, 1)] = v.y;
+      f[part_feature(p, 2)] = v.z;
+      f[part_feature(p, 3)] = v.w;
+    }
+    const RayData r = load_ray(rays, __float_as_uint(pos[i].w));
+    float h1[64], o2[16], in3[31], h3[64], h4[64], o5[3], sh[16];
+    dense_f32<32, 64, true>(W + kW1, f, h1);
+    dense_f32<64, 16, false>(W + kW2, h1, o2);
+    sh4(r.d, sh);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) in3[k] = sh[k];
+#pragma unroll
+    for (int k = 0; k < 15; ++k) in3[16 + k] = o2[1 + k];
+    dense_f32<31, 64, true>(W + kW3, in3, h3);
+    dense_f32<64, 64, true>(W + kW4, h3, h4);
+    dense_f32<64, 3, false>(W + kW5, h4, o5);
+    out[i] = make_float4(expf(o2[0]), sigmoidf_(o5[0]), sigmoidf_(o5[1]), sigmoidf_(o5[2]));
+  }
+}
+
+// FP16 MFMA MLPs: lane (g, j) of a 16-sample tile loads parts 2g and 2g+1 of
+// sample j -- the layer-1 B operand with K-slot s of group g holding feature
+// part_feature(2g + s/4, s%4) (baked into the W1 fragments).
+__global__ __launch_bounds__(256) void mlp_mixed(const float4 *__restrict__ pos,
+                                                 const uint2 *__restrict__ feat,
+                                                 const uint32_t *__restrict__ n_ptr,
+                                                 uint32_t stride,
+                                                 const RayData *__restrict__ rays,
+                                                 const half8 *__restrict__ wfrag,
+                                                 float4 *__restrict__ out) {
+  const uint32_t n = min(*n_ptr, stride);
   __shared__ half8 wl[20 * 64];
   for (int i = (int)threadIdx.x; i < 20 * 64; i += (int)blockDim.x) wl[i] = wfrag[i];
   __syncthreads();
   const int lane = lane_id(), wave = (int)(threadIdx.x >> 6);
   const int j = lane & 15, g = lane >> 4;
-  const half8 *wf = wl + lane;  // fragment m at wf[64 * m]
+  const half8 *wf = wl + lane;
   const float4v zero = {0.0f, 0.0f, 0.0f, 0.0f};
-  const LevelParams lp = level_params(g);
   const uint32_t n_tiles = (n + 15) / 16;
   for (uint32_t tile = blockIdx.x * kEvalWaves + wave; tile < n_tiles;
        tile += gridDim.x * kEvalWaves) {
     const uint32_t s = tile * 16 + (uint32_t)j;
-    float xc[3], d[3], f[8];
-    sample_geometry(samples, rays, min(s, n - 1), xc, d);
-    encode4(table, lp, xc, f);
+    const uint32_t sc = min(s, n - 1);  // n >= 1 inside the loop
+    const uint2 fa = feat[(size_t)(2 * g) * stride + sc];
+    const uint2 fb = feat[(size_t)(2 * g + 1) * stride + sc];
+    const RayData r = load_ray(rays, __float_as_uint(pos[sc].w));
+    const uint32_t words[4] = {fa.x, fa.y, fb.x, fb.y};
     half8 b1;
 #pragma unroll
-    for (int e = 0; e < 8; ++e) b1[e] = (_Float16)f[e];
-    const half4 shv = sh4_quarter(d, g);
+    for (int k = 0; k < 4; ++k) {
+      const half2v h = __builtin_bit_cast(half2v, words[k]);
+      b1[2 * k] = h.x;
+      b1[2 * k + 1] = h.y;
+    }
+    const half4 shv = sh4_quarter(r.d, g);
     float4v d1[4];
 #pragma unroll
     for (int m = 0; m < 4; ++m) d1[m] = mfma(wf[64 * m], b1, zero);
@@ -584,9 +541,10 @@ __global__ __launch_bounds__(256) void composite(
     uint32_t n_slots, const RayData *__restrict__ rays, RayAcc *acc,
     const uint32_t *__restrict__ counts, const uint32_t *__restrict__ offsets,
     const uint32_t *__restrict__ samples, const float4 *__restrict__ sout,
-    int last, OutPtrs o) {
+    const uint32_t *__restrict__ n_ptr, uint32_t capacity, int first, int last, OutPtrs o) {
   const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
   if (slot >= n_slots) return;
+  if (*n_ptr > capacity) return;  // overflowed pass: the host re-renders
   RayAcc a = acc[slot];
   const uint32_t cnt = counts[slot];
   if (!a.done && cnt) {
@@ -609,8 +567,9 @@ __global__ __launch_bounds__(256) void composite(
         break;
       }
     }
-    acc[slot] = a;
+    if (!last) acc[slot] = a;
   }
+  (void)first;
   if (last && a.pad != 0xffffffffu) {
     const uint32_t p = a.pad;
     const float c[3] = {a.r, a.g, a.b};
@@ -638,8 +597,10 @@ void build_mlp_fragments(const float *w, _Float16 *frag) {
   for (int lane = 0; lane < 64; ++lane) {
     const int i = lane & 15, g = lane >> 4;
     for (int s = 0; s < 8; ++s) {
-      // layer 1: 32 features (natural K order) -> 64
-      for (int m = 0; m < 4; ++m) put(m, lane, s, w[kW1 + (16 * m + i) * 32 + 8 * g + s]);
+      // layer 1: 32 features -> 64; K-slot (g, s) holds feature
+      // part_feature(2g + s/4, s%4) as delivered by encode_parts
+      for (int m = 0; m < 4; ++m)
+        put(m, lane, s, w[kW1 + (16 * m + i) * 32 + part_feature(2 * g + s / 4, s % 4)]);
       // layer 2: 64 -> 16, two K-chunks
       for (int c = 0; c < 2; ++c) put(4 + c, lane, s, w[kW2 + i * 64 + hid(c, g, s)]);
       // layer 3: [density outputs 4g..4g+3 | SH 4g..4g+3] -> 64; density
@@ -697,7 +658,7 @@ void scene_free(SceneDev &s) {
 
 void render_work_free(RenderWork &w) {
   for (DevBuf *b : {&w.views, &w.rays, &w.acc, &w.counts, &w.offsets, &w.samples,
-                    &w.sample_out, &w.out, &w.total})
+                    &w.sample_out, &w.sample_pos, &w.feat, &w.out, &w.total})
     b->release();
   if (w.total_host) hipHostFree(w.total_host);
   w.total_host = nullptr;
@@ -778,58 +739,70 @@ int render_setup(RenderWork &w, const nes_camera *views, int n_views, hipStream_
   return NES_OK;
 }
 
-int render_launch_count(const SceneDev &scene, RenderWork &w, int k0, int k1, bool first,
-                        hipStream_t stream) {
-  const unsigned blocks = (unsigned)((w.n_slots + 255) / 256);
-  uint32_t *total = w.total.as<uint32_t>();
-  zero_u32<<<1, 64, 0, stream>>>(total, 2);
-  march_count<<<blocks, 256, 0, stream>>>(w.views.as<ViewDesc>(), w.n_views, w.n_slots, scene.occ,
-                                         w.rays.as<RayData>(), w.acc.as<RayAcc>(),
-                                         w.counts.as<uint32_t>(), total, k0, k1, first ? 1 : 0);
-  NES_HIP_TRY(hipGetLastError());
-  NES_HIP_TRY(hipMemcpyAsync(w.total_host, total, 4, hipMemcpyDeviceToHost, stream));
+int render_reserve_samples(RenderWork &w, int precision, size_t capacity) {
+  if (capacity > 0xffffffffull) capacity = 0xffffffffull;
+  const size_t fbytes = precision == NES_PRECISION_FP32 ? sizeof(float4) : sizeof(uint2);
+  NES_TRY(w.samples.reserve(sizeof(uint32_t) * (capacity + 1)));
+  NES_TRY(w.sample_out.reserve(sizeof(float4) * (capacity + 1)));
+  NES_TRY(w.sample_pos.reserve(sizeof(float4) * (capacity + 1)));
+  NES_TRY(w.feat.reserve(fbytes * kParts * (capacity + 1)));
+  w.capacity = (uint32_t)capacity;
+  w.capacity_precision = precision;
   return NES_OK;
 }
 
-int render_launch_rest(const SceneDev &scene, RenderWork &w, int precision, int k0, int k1,
-                       bool last, uint32_t n, hipStream_t stream, hipEvent_t ev_eval_begin,
+int render_launch_pass(const SceneDev &scene, RenderWork &w, int precision, int k0, int k1,
+                       bool first, bool last, hipStream_t stream, hipEvent_t ev_eval_begin,
                        hipEvent_t ev_eval_end) {
   const unsigned blocks = (unsigned)((w.n_slots + 255) / 256);
-  uint32_t *total = w.total.as<uint32_t>();
-  NES_TRY(w.samples.reserve(sizeof(uint32_t) * ((size_t)n + 1)));
-  NES_TRY(w.sample_out.reserve(sizeof(float4) * ((size_t)n + 1)));
-  march_fill<<<blocks, 256, 0, stream>>>(w.n_slots, scene.occ, w.rays.as<RayData>(),
+  uint32_t *cursor = w.total.as<uint32_t>();
+  const uint32_t cap = w.capacity;
+  zero_u32<<<1, 64, 0, stream>>>(cursor, 2);
+  march_emit<<<blocks, 256, 0, stream>>>(w.views.as<ViewDesc>(), w.n_views, w.n_slots, scene.occ,
+                                        w.rays.as<RayData>(), w.acc.as<RayAcc>(),
                                         w.counts.as<uint32_t>(), w.offsets.as<uint32_t>(),
-                                        w.samples.as<uint32_t>(), total + 1, k0, k1);
+                                        w.samples.as<uint32_t>(), w.sample_pos.as<float4>(),
+                                        cursor, cap, k0, k1, first ? 1 : 0);
   NES_HIP_TRY(hipGetLastError());
   if (ev_eval_begin) NES_HIP_TRY(hipEventRecord(ev_eval_begin, stream));
-  if (n) {
-    const int dev_cu = 256;
-    if (precision == NES_PRECISION_FP32) {
-      const uint64_t chunks = (n + 63) / 64;
-      const unsigned eb = (unsigned)std::min<uint64_t>((chunks + kEvalWaves - 1) / kEvalWaves,
-                                                       (uint64_t)dev_cu * 4);
-      eval_fp32<<<eb, 256, 0, stream>>>(w.samples.as<uint32_t>(), n, w.rays.as<RayData>(),
-                                        scene.table_f32, scene.weights_f32,
-                                        w.sample_out.as<float4>());
-    } else {
-      const uint64_t tiles = (n + 15) / 16;
-      const unsigned eb = (unsigned)std::min<uint64_t>((tiles + kEvalWaves - 1) / kEvalWaves,
-                                                       (uint64_t)dev_cu * 8);
-      eval_mixed<<<eb, 256, 0, stream>>>(w.samples.as<uint32_t>(), n, w.rays.as<RayData>(),
-                                         scene.table_f16,
-                                         reinterpret_cast<const half8 *>(scene.wfrag),
-                                         w.sample_out.as<float4>());
-    }
+  // fixed grids with grid-stride loops: the sample count stays on the device
+  const uint32_t bpp = 512;
+  if (precision == NES_PRECISION_FP32) {
+    encode_parts<float><<<bpp * kParts, 256, 0, stream>>>(w.sample_pos.as<float4>(), cursor, cap,
+                                                           scene.table_f32, w.feat.as<float4>(),
+                                                           bpp);
     NES_HIP_TRY(hipGetLastError());
+    mlp_fp32<<<256 * 4, 256, 0, stream>>>(w.sample_pos.as<float4>(), w.feat.as<float4>(), cursor,
+                                          cap, w.rays.as<RayData>(), scene.weights_f32,
+                                          w.sample_out.as<float4>());
+  } else {
+    encode_parts<_Float16><<<bpp * kParts, 256, 0, stream>>>(w.sample_pos.as<float4>(), cursor,
+                                                              cap, scene.table_f16,
+                                                              w.feat.as<uint2>(), bpp);
+    NES_HIP_TRY(hipGetLastError());
+    mlp_mixed<<<256 * 8, 256, 0, stream>>>(w.sample_pos.as<float4>(), w.feat.as<uint2>(), cursor,
+                                           cap, w.rays.as<RayData>(),
+                                           reinterpret_cast<const half8 *>(scene.wfrag),
+                                           w.sample_out.as<float4>());
   }
+  NES_HIP_TRY(hipGetLastError());
   if (ev_eval_end) NES_HIP_TRY(hipEventRecord(ev_eval_end, stream));
   composite<<<blocks, 256, 0, stream>>>(w.n_slots, w.rays.as<RayData>(), w.acc.as<RayAcc>(),
                                        w.counts.as<uint32_t>(), w.offsets.as<uint32_t>(),
                                        w.samples.as<uint32_t>(), w.sample_out.as<float4>(),
-                                       last ? 1 : 0, out_ptrs(w));
+                                       cursor, cap, first ? 1 : 0, last ? 1 : 0, out_ptrs(w));
   NES_HIP_TRY(hipGetLastError());
+  NES_HIP_TRY(hipMemcpyAsync(w.total_host, cursor, 4, hipMemcpyDeviceToHost, stream));
   return NES_OK;
+}
+
+// Initial sample capacity: four samples per ray slot, or what the last render
+// of this work area needed.
+int render_prepare_capacity(RenderWork &w, int precision) {
+  size_t want = std::max<size_t>((size_t)w.n_slots * 4, 1u << 20);
+  if (w.capacity >= want && w.capacity_precision == precision) return NES_OK;
+  want = std::max<size_t>(want, w.capacity);
+  return render_reserve_samples(w, precision, want);
 }
 
 int render_enqueue(const SceneDev &scene, RenderWork &w, int precision,
@@ -839,21 +812,30 @@ int render_enqueue(const SceneDev &scene, RenderWork &w, int precision,
   if (steps_per_pass < 0) return NES_EINVAL;
   const int spp = steps_per_pass == 0 ? kMaxSteps : std::min(steps_per_pass, kMaxSteps);
   NES_TRY(render_setup(w, views, n_views, stream));
+  NES_TRY(render_prepare_capacity(w, precision));
   double ms_march = 0, ms_eval = 0, ms_comp = 0;
   uint64_t n_samples = 0;
   NES_HIP_TRY(hipEventRecord(w.ev[0], stream));
   const int passes = (kMaxSteps + spp - 1) / spp;
   for (int p = 0; p < passes; ++p) {
     const int k0 = p * spp, k1 = std::min(kMaxSteps, k0 + spp);
-    NES_HIP_TRY(hipEventRecord(w.ev[1], stream));
-    NES_TRY(render_launch_count(scene, w, k0, k1, p == 0, stream));
-    NES_HIP_TRY(hipStreamSynchronize(stream));
-    const uint32_t n = w.total_host[0];
-    n_samples += n;
-    NES_TRY(render_launch_rest(scene, w, precision, k0, k1, p == passes - 1, n, stream, w.ev[2],
-                               w.ev[3]));
-    NES_HIP_TRY(hipEventRecord(w.ev[4], stream));
-    NES_HIP_TRY(hipEventSynchronize(w.ev[4]));
+    for (int attempt = 0;; ++attempt) {
+      NES_HIP_TRY(hipEventRecord(w.ev[1], stream));
+      NES_TRY(render_launch_pass(scene, w, precision, k0, k1, p == 0, p == passes - 1, stream,
+                                 w.ev[2], w.ev[3]));
+      NES_HIP_TRY(hipEventRecord(w.ev[4], stream));
+      NES_HIP_TRY(hipEventSynchronize(w.ev[4]));
+      const uint32_t n = w.total_host[0];
+      if (n <= w.capacity) {
+        n_samples += n;
+        break;
+      }
+      if (attempt > 0) {
+        set_last_error("sample capacity still exceeded after regrowth");
+        return NES_ENOMEM;
+      }
+      NES_TRY(render_reserve_samples(w, precision, (size_t)n + n / 8 + 1024));
+    }
     float a = 0, b = 0, c = 0;
     hipEventElapsedTime(&a, w.ev[1], w.ev[2]);
     hipEventElapsedTime(&b, w.ev[2], w.ev[3]);

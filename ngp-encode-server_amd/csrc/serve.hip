@@ -15,11 +15,12 @@
 //     admission is by memory.  Batches on different slots overlap on the
 //     device, and a request arriving while others render joins the next free
 //     slot without waiting for them (continuous batching);
-//   * each batch runs the render stages (march count -> async total read-back
-//     -> fill / eval / composite) then frame prep of every view straight from
-//     the device RGB24 (no host round trip) and an async copy of the YUV420P
-//     planes into pinned host memory.  The scheduler polls events, so it never
-//     blocks on one batch while another could progress.
+//   * each batch is one asynchronous launch sequence: the render pass (march,
+//     encode, MLPs, composite -- the sample count never leaves the device),
+//     frame prep of every view straight from the device RGB24, and an async
+//     copy of the YUV420P planes into pinned host memory.  The scheduler polls
+//     completion events, so it never blocks on one batch while another could
+//     progress.
 // All per-ray work is independent of batch composition, so a request's frame
 // is bit-identical alone or in any mix.
 #include <algorithm>
@@ -106,10 +107,10 @@ struct Slot {
   hipStream_t stream = nullptr;
   RenderWork work;
   DevBuf yuv;
-  hipEvent_t ev_count = nullptr, ev_done = nullptr;
+  hipEvent_t ev_done = nullptr;
   std::vector<std::shared_ptr<Request>> reqs;
   std::vector<nes_camera> views;
-  int stage = 0;  // 0 idle, 1 counting, 2 rendering
+  int stage = 0;  // 0 idle, 2 in flight
   Clock::time_point launched;
 };
 
@@ -228,30 +229,35 @@ static void finish_batch(nes_server *s, Slot &sl, int status) {
   s->cv_done.notify_all();
 }
 
+// Launch the whole batch (render pass, frame prep, D2H) and its done event.
+static int launch_batch(nes_server *s, Slot &sl) {
+  const SceneDev &scene = s->ctx->scene;
+  NES_TRY(render_launch_pass(scene, sl.work, s->cfg.precision, 0, kMaxSteps, true, true,
+                             sl.stream, nullptr, nullptr));
+  NES_TRY(launch_output(s, sl));
+  NES_HIP_TRY(hipEventRecord(sl.ev_done, sl.stream));
+  return NES_OK;
+}
+
 static void scheduler_loop(nes_server *s) {
   hipSetDevice(0);
-  const SceneDev &scene = s->ctx->scene;
   while (true) {
     bool progressed = false;
     for (auto &slp : s->slots) {
       Slot &sl = *slp;
-      if (sl.stage == 1 && hipEventQuery(sl.ev_count) == hipSuccess) {
-        const uint32_t n = sl.work.total_host[0];
-        int rc = render_launch_rest(scene, sl.work, s->cfg.precision, 0, kMaxSteps, true, n,
-                                    sl.stream, nullptr, nullptr);
-        if (rc == NES_OK) rc = launch_output(s, sl);
-        if (rc == NES_OK && hipEventRecord(sl.ev_done, sl.stream) != hipSuccess) rc = NES_EIO;
-        if (rc != NES_OK) {
-          s->last_error = rc;
-          hipStreamSynchronize(sl.stream);
-          finish_batch(s, sl, rc);
-        } else {
-          sl.stage = 2;
-        }
-        progressed = true;
-      } else if (sl.stage == 2 && hipEventQuery(sl.ev_done) == hipSuccess) {
+      if (sl.stage != 2 || hipEventQuery(sl.ev_done) != hipSuccess) continue;
+      progressed = true;
+      const uint32_t n = sl.work.total_host[0];
+      if (n > sl.work.capacity) {
+        // the pass dropped samples beyond capacity: grow and run it again
+        int rc = render_reserve_samples(sl.work, s->cfg.precision, (size_t)n + n / 8 + 1024);
+        if (rc == NES_OK) rc = launch_batch(s, sl);
+        if (rc == NES_OK) continue;
+        s->last_error = rc;
+        hipStreamSynchronize(sl.stream);
+        finish_batch(s, sl, rc);
+      } else {
         finish_batch(s, sl, NES_OK);
-        progressed = true;
       }
     }
     // admit waiting requests into free slots
@@ -264,15 +270,15 @@ static void scheduler_loop(nes_server *s) {
         if (sl.reqs.empty()) continue;
         lk.unlock();
         int rc = render_setup(sl.work, sl.views.data(), (int)sl.views.size(), sl.stream);
-        if (rc == NES_OK) rc = render_launch_count(scene, sl.work, 0, kMaxSteps, true, sl.stream);
-        if (rc == NES_OK && hipEventRecord(sl.ev_count, sl.stream) != hipSuccess) rc = NES_EIO;
+        if (rc == NES_OK) rc = render_prepare_capacity(sl.work, s->cfg.precision);
+        if (rc == NES_OK) rc = launch_batch(s, sl);
         sl.launched = Clock::now();
         if (rc != NES_OK) {
           s->last_error = rc;
           hipStreamSynchronize(sl.stream);
           finish_batch(s, sl, rc);
         } else {
-          sl.stage = 1;
+          sl.stage = 2;
         }
         lk.lock();
         progressed = true;
@@ -318,7 +324,6 @@ int nes_server_create(nes_ctx *ctx, const nes_server_config *cfg, nes_server **o
   for (uint32_t i = 0; i < n_slots; ++i) {
     auto sl = std::make_unique<Slot>();
     NES_HIP_TRY(hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking));
-    NES_HIP_TRY(hipEventCreateWithFlags(&sl->ev_count, hipEventDisableTiming));
     NES_HIP_TRY(hipEventCreateWithFlags(&sl->ev_done, hipEventDisableTiming));
     s->slots.push_back(std::move(sl));
   }
@@ -346,7 +351,6 @@ int nes_server_destroy(nes_server *s) {
     hipStreamSynchronize(sl->stream);
     render_work_free(sl->work);
     sl->yuv.release();
-    hipEventDestroy(sl->ev_count);
     hipEventDestroy(sl->ev_done);
     hipStreamDestroy(sl->stream);
   }
