@@ -156,6 +156,54 @@ int nes_gray8_to_yuv420p(const uint8_t *gray, int w, int h, int g_stride,
                          uint8_t *y, int y_stride, uint8_t *u, int u_stride,
                          uint8_t *v, int v_stride, int sws_mode);
 
+/* -------------------------------------------------------------- training */
+/* Differentiable render step for the same scene representation: L2
+ * photometric loss, hand-written backward through compositing, the MLPs
+ * (FP32 MFMA, activations recomputed per 16-sample tile rather than stored)
+ * and an FP32 atomic scatter-add into the hash-table gradient, then Adam.
+ * The reference ships no trainer (its renderer loads a trained snapshot);
+ * this is the training leg of the render path above. */
+typedef struct nes_trainer nes_trainer;
+
+typedef struct nes_train_config {
+  uint32_t rays_per_batch; /* 0 = 2^18 */
+  uint32_t n_poses;        /* target views in pose_c2w / targets */
+  uint32_t width, height;  /* target image size */
+  float fov_y_deg;         /* 0 = 60 */
+  uint64_t seed;           /* ray sampling: splitmix64 stream 1000 + step */
+  float lr, beta1, beta2, eps; /* 0 = 1e-2, 0.9, 0.99, 1e-15 */
+} nes_train_config;
+
+typedef struct nes_train_stats {
+  double ms;        /* device time of the step */
+  uint64_t rays, samples, step;
+} nes_train_stats;
+
+/* init: starting parameters and the (fixed) occupancy grid.  pose_c2w:
+ * n_poses row-major 3x4 matrices; targets: n_poses x H x W x 3 f32 colours.
+ * Both may be NULL for a trainer driven only by explicit rays. */
+int nes_trainer_create(const nes_scene_desc *init, const nes_train_config *cfg,
+                       const float *pose_c2w, const float *targets,
+                       nes_trainer **out);
+int nes_trainer_destroy(nes_trainer *trainer);
+/* One step: sample the batch of the current step, forward, loss, backward,
+ * Adam.  loss receives the batch loss before the update. */
+int nes_trainer_step(nes_trainer *trainer, float *loss, nes_train_stats *stats);
+/* Loss and gradients of step `step`'s batch without updating (host outputs
+ * may be NULL). */
+int nes_trainer_batch_grad(nes_trainer *trainer, uint64_t step, float *loss,
+                           float *grad_table, float *grad_weights);
+int nes_trainer_get_params(nes_trainer *trainer, float *table, float *weights);
+int nes_trainer_set_params(nes_trainer *trainer, const float *table,
+                           const float *weights);
+/* Raw differentiable render of explicit rays (n x 3 origins, unit
+ * directions) with the current parameters: rgb is n x 3 ... */
+int nes_trainer_forward(nes_trainer *trainer, const float *ray_o,
+                        const float *ray_d, uint32_t n, float *rgb);
+/* ... and the gradient of sum(d_rgb * rgb) for the rays of the last forward. */
+int nes_trainer_backward(nes_trainer *trainer, const float *d_rgb,
+                         float *grad_table, float *grad_weights);
+
 /* --------------------------------------------------------------- serving */
 /* Continuous-batching server: replaces the reference's per-connection loop
  * of one blocking FrameRequest in flight (socket_client_thread,

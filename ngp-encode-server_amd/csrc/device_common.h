@@ -1,0 +1,240 @@
+// device_common.h -- device helpers shared by the render and training
+// kernels: FP32 sample geometry, SH basis, table access and the
+// XCD-partitioned level layout.
+#pragma once
+
+#include "render.h"
+
+namespace nes {
+
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef _Float16 half4 __attribute__((ext_vector_type(4)));
+typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+typedef float float4v __attribute__((ext_vector_type(4)));
+
+// ------------------------------------------------------------- geometry
+__device__ __forceinline__ void make_ray(const ViewDesc &v, uint32_t px,
+                                         uint32_t py, float o[3], float d[3]) {
+#pragma clang fp contract(off)
+  const float u = (((float)px + 0.5f) - 0.5f * (float)v.width) / v.focal;
+  const float w = (((float)py + 0.5f) - 0.5f * (float)v.height) / v.focal;
+  float dw[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    dw[i] = fmaf(v.c2w[4 * i + 0], u, fmaf(v.c2w[4 * i + 1], w, v.c2w[4 * i + 2]));
+  const float n = sqrtf(fmaf(dw[0], dw[0], fmaf(dw[1], dw[1], dw[2] * dw[2])));
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    d[i] = dw[i] / n;
+    o[i] = v.c2w[4 * i + 3];
+  }
+}
+
+__device__ __forceinline__ float step_t(int k, float t_start) {
+  return fmaf((float)k + 0.5f, kDelta, t_start);
+}
+
+// Returns the number of candidate steps (<= 128) inside [0,1]^3.
+__device__ __forceinline__ uint32_t ray_box(const float o[3], const float d[3],
+                                            float *t_start) {
+#pragma clang fp contract(off)
+  float tn = -INFINITY, tf = INFINITY;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if (d[a] == 0.0f) {
+      if (o[a] < 0.0f || o[a] > 1.0f) return 0;
+      continue;
+    }
+    const float t0 = (0.0f - o[a]) / d[a];
+    const float t1 = (1.0f - o[a]) / d[a];
+    const float lo = t0 < t1 ? t0 : t1;
+    const float hi = t0 < t1 ? t1 : t0;
+    tn = lo > tn ? lo : tn;
+    tf = hi < tf ? hi : tf;
+  }
+  const float ts = tn > 0.0f ? tn : 0.0f;
+  *t_start = ts;
+  if (!(tf > ts)) return 0;
+  uint32_t n = 0;
+  while (n < (uint32_t)kMaxSteps && step_t((int)n, ts) < tf) ++n;
+  return n;
+}
+
+__device__ __forceinline__ bool occupied(const uint32_t *__restrict__ occ,
+                                         const float x[3]) {
+  int c[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int ci = (int)floorf(x[a] * 128.0f);
+    c[a] = min(max(ci, 0), kOccRes - 1);
+  }
+  const uint32_t cell = (uint32_t)(c[0] + kOccRes * (c[1] + kOccRes * c[2]));
+  return (__ldg(occ + (cell >> 5)) >> (cell & 31)) & 1u;
+}
+
+__device__ __forceinline__ void step_pos(const RayData &r, int k, float x[3],
+                                         float *t) {
+  const float tt = step_t(k, r.t_start);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) x[a] = fmaf(tt, r.d[a], r.o[a]);
+  *t = tt;
+}
+
+__device__ __forceinline__ RayData load_ray(const RayData *rays, uint32_t slot) {
+  const float4 *p = reinterpret_cast<const float4 *>(rays + slot);
+  const float4 a = p[0], b = p[1];
+  RayData r;
+  r.o[0] = a.x; r.o[1] = a.y; r.o[2] = a.z;
+  r.d[0] = a.w; r.d[1] = b.x; r.d[2] = b.y;
+  r.t_start = b.z;
+  r.n_steps = __float_as_uint(b.w);
+  return r;
+}
+
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+
+// --------------------------------------------------------------- encoding
+__device__ __forceinline__ void sh4(const float d[3], float sh[16]) {
+  const float x = d[0], y = d[1], z = d[2];
+  const float xx = x * x, yy = y * y, zz = z * z;
+  sh[0] = 0.28209479177387814f;
+  sh[1] = -0.48860251190291987f * y;
+  sh[2] = 0.48860251190291987f * z;
+  sh[3] = -0.48860251190291987f * x;
+  sh[4] = 1.0925484305920792f * x * y;
+  sh[5] = -1.0925484305920792f * y * z;
+  sh[6] = 0.94617469575755997f * zz - 0.31539156525251999f;
+  sh[7] = -1.0925484305920792f * x * z;
+  sh[8] = 0.54627421529603959f * xx - 0.54627421529603959f * yy;
+  sh[9] = 0.59004358992664352f * y * (-3.0f * xx + yy);
+  sh[10] = 2.8906114426405538f * x * y * z;
+  sh[11] = 0.45704579946446572f * y * (1.0f - 5.0f * zz);
+  sh[12] = 0.3731763325901154f * z * (5.0f * zz - 3.0f);
+  sh[13] = 0.45704579946446572f * x * (1.0f - 5.0f * zz);
+  sh[14] = 1.4453057213202769f * z * (xx - yy);
+  sh[15] = 0.59004358992664352f * x * (-xx + 3.0f * yy);
+}
+
+template <class Tab> struct RawFeat;
+template <> struct RawFeat<float> {
+  typedef float2 type;
+  static __device__ __forceinline__ float2 load(const float *__restrict__ t, uint32_t e) {
+    return __ldg(reinterpret_cast<const float2 *>(t) + e);
+  }
+  static __device__ __forceinline__ float2 unpack(float2 v) { return v; }
+};
+template <> struct RawFeat<_Float16> {
+  typedef uint32_t type;
+  static __device__ __forceinline__ uint32_t load(const _Float16 *__restrict__ t, uint32_t e) {
+    return __ldg(reinterpret_cast<const uint32_t *>(t) + e);
+  }
+  static __device__ __forceinline__ float2 unpack(uint32_t bits) {
+    const half2v h = __builtin_bit_cast(half2v, bits);
+    return make_float2((float)h.x, (float)h.y);
+  }
+};
+
+__device__ __forceinline__ float sigmoidf_(float v) { return 1.0f / (1.0f + expf(-v)); }
+
+// Orders a wavefront's own LDS writes before its later LDS reads of other
+// lanes' data (LDS ops of one wavefront complete in order).
+__device__ __forceinline__ void wave_lds_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+
+// ------------------------------------------------- XCD-partitioned levels
+constexpr int kParts = 8;
+
+struct PartLevels {
+  float scale[2];
+  uint32_t n1[2], n1sq[2], off[2];
+  bool dense[2];
+};
+
+__device__ __forceinline__ PartLevels part_levels(int part) {
+  PartLevels p;
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int l = q == 0 ? part : 15 - part;
+    int res = 0;
+    uint32_t off = 0;
+#pragma unroll
+    for (int c = 0; c < 16; ++c)
+      if (l == c) {
+        res = kLevelRes[c];
+        off = kLevelOffset[c];
+      }
+    p.scale[q] = (float)res;
+    p.n1[q] = (uint32_t)res + 1u;
+    p.n1sq[q] = p.n1[q] * p.n1[q];
+    p.off[q] = off;
+    p.dense[q] = l < kDenseLevels;
+  }
+  return p;
+}
+
+template <class Tab>
+__device__ __forceinline__ float2 encode_one(const Tab *__restrict__ table, const PartLevels &lp,
+                                             int q, const float xc[3]) {
+  uint32_t i0[3];
+  float fr[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float pp = xc[a] * lp.scale[q];
+    const float fl = fminf(floorf(pp), lp.scale[q] - 1.0f);
+    i0[a] = (uint32_t)fl;
+    fr[a] = pp - fl;
+  }
+  typename RawFeat<Tab>::type v[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const uint32_t x = i0[0] + (c & 1), y = i0[1] + ((c >> 1) & 1), z = i0[2] + ((c >> 2) & 1);
+    const uint32_t dense = x + y * lp.n1[q] + z * lp.n1sq[q];
+    const uint32_t hash = (x ^ (y * kPrime1) ^ (z * kPrime2)) & ((1u << kLog2T) - 1u);
+    v[c] = RawFeat<Tab>::load(table, lp.off[q] + (lp.dense[q] ? dense : hash));
+  }
+  const float gx = 1.0f - fr[0], gy = 1.0f - fr[1], gz = 1.0f - fr[2];
+  float a0 = 0.0f, a1 = 0.0f;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const float w = ((c & 1) ? fr[0] : gx) * (((c >> 1) & 1) ? fr[1] : gy) *
+                    (((c >> 2) & 1) ? fr[2] : gz);
+    const float2 fv = RawFeat<Tab>::unpack(v[c]);
+    a0 = fmaf(w, fv.x, a0);
+    a1 = fmaf(w, fv.y, a1);
+  }
+  return make_float2(a0, a1);
+}
+
+template <class Tab> struct PartOut;
+template <> struct PartOut<float> {
+  typedef float4 type;
+  static __device__ __forceinline__ float4 pack(float2 a, float2 b) { return make_float4(a.x, a.y, b.x, b.y); }
+};
+template <> struct PartOut<_Float16> {
+  typedef uint2 type;
+  static __device__ __forceinline__ uint2 pack(float2 a, float2 b) {
+    half2v ha, hb;
+    ha.x = (_Float16)a.x; ha.y = (_Float16)a.y;
+    hb.x = (_Float16)b.x; hb.y = (_Float16)b.y;
+    return make_uint2(__builtin_bit_cast(uint32_t, ha), __builtin_bit_cast(uint32_t, hb));
+  }
+};
+
+// Feature index of part p, component c (the order encode_parts writes).
+__host__ __device__ constexpr int part_feature(int p, int c) {
+  return c < 2 ? 2 * p + c : 2 * (15 - p) + (c - 2);
+}
+
+
+}  // namespace nes

@@ -79,6 +79,12 @@ int render_launch_pass(const SceneDev &scene, RenderWork &w, int precision, int 
                        bool first, bool last, hipStream_t stream, hipEvent_t ev_eval_begin,
                        hipEvent_t ev_eval_end);
 
+// Single-stage launchers (training step): march_emit over prepared rays,
+// FP32 encode_parts, composite.
+int launch_march_emit(const uint32_t *occ, RenderWork &w, int k0, int k1, hipStream_t stream);
+int launch_encode_f32(const float *table, RenderWork &w, hipStream_t stream);
+int launch_composite(RenderWork &w, bool last, hipStream_t stream);
+
 // Enqueue a full render of views on stream; returns after the sample count
 // read-backs (one host sync per pass).  Outputs land in work.out.
 int render_enqueue(const SceneDev &scene, RenderWork &work, int precision,

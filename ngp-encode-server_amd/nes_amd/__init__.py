@@ -91,6 +91,17 @@ class SimConfig(C.Structure):
                 ("fov_min", C.c_double), ("fov_max", C.c_double), ("stereo", C.c_int)]
 
 
+class TrainConfig(C.Structure):
+    _fields_ = [("rays_per_batch", C.c_uint32), ("n_poses", C.c_uint32), ("width", C.c_uint32),
+                ("height", C.c_uint32), ("fov_y_deg", C.c_float), ("seed", C.c_uint64),
+                ("lr", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float)]
+
+
+class TrainStats(C.Structure):
+    _fields_ = [("ms", C.c_double), ("rays", C.c_uint64), ("samples", C.c_uint64),
+                ("step", C.c_uint64)]
+
+
 # C-ABI symbols declared in include/nes_render.h (checked by the CPU tests).
 _SIGS = {}
 
@@ -123,6 +134,17 @@ _sig("nes_gray8_to_yuv420p", C.c_int, _P(C.c_uint8), C.c_int, C.c_int, C.c_int, 
      C.c_int, _P(C.c_uint8), C.c_int, _P(C.c_uint8), C.c_int, C.c_int)
 _sig("nes_sws_vchroma_filter", C.c_int, C.c_int, _P(C.c_int16), _P(C.c_int32), C.c_int,
      _P(C.c_int))
+_sig("nes_trainer_create", C.c_int, _P(SceneDesc), _P(TrainConfig), _P(C.c_float), _P(C.c_float),
+     _P(C.c_void_p))
+_sig("nes_trainer_destroy", C.c_int, C.c_void_p)
+_sig("nes_trainer_step", C.c_int, C.c_void_p, _P(C.c_float), _P(TrainStats))
+_sig("nes_trainer_batch_grad", C.c_int, C.c_void_p, C.c_uint64, _P(C.c_float), _P(C.c_float),
+     _P(C.c_float))
+_sig("nes_trainer_get_params", C.c_int, C.c_void_p, _P(C.c_float), _P(C.c_float))
+_sig("nes_trainer_set_params", C.c_int, C.c_void_p, _P(C.c_float), _P(C.c_float))
+_sig("nes_trainer_forward", C.c_int, C.c_void_p, _P(C.c_float), _P(C.c_float), C.c_uint32,
+     _P(C.c_float))
+_sig("nes_trainer_backward", C.c_int, C.c_void_p, _P(C.c_float), _P(C.c_float), _P(C.c_float))
 _sig("nes_server_create", C.c_int, C.c_void_p, _P(ServerConfig), _P(C.c_void_p))
 _sig("nes_server_destroy", C.c_int, C.c_void_p)
 _sig("nes_server_submit", C.c_int, C.c_void_p, C.c_uint32, _P(Camera), C.c_int, _P(C.c_uint64))
@@ -415,6 +437,82 @@ class Server:
     def close(self):
         if self.handle:
             check(lib().nes_server_destroy(self.handle), "nes_server_destroy")
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+# --------------------------------------------------------------- training
+
+class Trainer:
+    """Differentiable render + Adam over the hash-grid scene (nes_trainer)."""
+
+    def __init__(self, table, weights, occ, poses=None, targets=None, width=0, height=0,
+                 rays_per_batch=1 << 18, fov=60.0, seed=0, lr=1e-2, beta1=0.9, beta2=0.99,
+                 eps=1e-15):
+        keep = (np.ascontiguousarray(table, np.float32), np.ascontiguousarray(weights, np.float32),
+                np.ascontiguousarray(occ, np.uint32))
+        desc = SceneDesc(_ptr(keep[0], C.c_float), _ptr(keep[1], C.c_float),
+                         _ptr(keep[2], C.c_uint32))
+        n_poses = 0 if poses is None else len(poses)
+        cfg = TrainConfig(rays_per_batch, n_poses, width, height, fov, seed, lr, beta1, beta2, eps)
+        p = None if poses is None else np.ascontiguousarray(poses, np.float32).reshape(-1)
+        t = None if targets is None else np.ascontiguousarray(targets, np.float32).reshape(-1)
+        h = C.c_void_p()
+        check(lib().nes_trainer_create(C.byref(desc), C.byref(cfg), _ptr(p, C.c_float),
+                                       _ptr(t, C.c_float), C.byref(h)), "nes_trainer_create")
+        self.handle = h
+
+    def step(self):
+        loss = C.c_float()
+        st = TrainStats()
+        check(lib().nes_trainer_step(self.handle, C.byref(loss), C.byref(st)), "nes_trainer_step")
+        return loss.value, st
+
+    def batch_grad(self, step):
+        loss = C.c_float()
+        gt = np.empty(TABLE_ENTRIES * 2, np.float32)
+        gw = np.empty(MLP_WEIGHTS, np.float32)
+        check(lib().nes_trainer_batch_grad(self.handle, step, C.byref(loss), _ptr(gt, C.c_float),
+                                           _ptr(gw, C.c_float)), "nes_trainer_batch_grad")
+        return loss.value, gt, gw
+
+    def get_params(self):
+        t = np.empty(TABLE_ENTRIES * 2, np.float32)
+        w = np.empty(MLP_WEIGHTS, np.float32)
+        check(lib().nes_trainer_get_params(self.handle, _ptr(t, C.c_float), _ptr(w, C.c_float)),
+              "nes_trainer_get_params")
+        return t, w
+
+    def set_params(self, table=None, weights=None):
+        t = None if table is None else np.ascontiguousarray(table, np.float32)
+        w = None if weights is None else np.ascontiguousarray(weights, np.float32)
+        check(lib().nes_trainer_set_params(self.handle, _ptr(t, C.c_float), _ptr(w, C.c_float)),
+              "nes_trainer_set_params")
+
+    def forward(self, ray_o, ray_d):
+        o = np.ascontiguousarray(ray_o, np.float32)
+        d = np.ascontiguousarray(ray_d, np.float32)
+        rgb = np.empty((len(o), 3), np.float32)
+        check(lib().nes_trainer_forward(self.handle, _ptr(o, C.c_float), _ptr(d, C.c_float), len(o),
+                                        _ptr(rgb, C.c_float)), "nes_trainer_forward")
+        return rgb
+
+    def backward(self, d_rgb):
+        g = np.ascontiguousarray(d_rgb, np.float32)
+        gt = np.empty(TABLE_ENTRIES * 2, np.float32)
+        gw = np.empty(MLP_WEIGHTS, np.float32)
+        check(lib().nes_trainer_backward(self.handle, _ptr(g, C.c_float), _ptr(gt, C.c_float),
+                                         _ptr(gw, C.c_float)), "nes_trainer_backward")
+        return gt, gw
+
+    def close(self):
+        if self.handle:
+            check(lib().nes_trainer_destroy(self.handle), "nes_trainer_destroy")
             self.handle = None
 
     def __enter__(self):

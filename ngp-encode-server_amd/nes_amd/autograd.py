@@ -1,0 +1,29 @@
+"""Thin torch.autograd.Function over the trainer's raw forward/backward.
+
+The GPU work happens inside libnes_render.so (nes_trainer_forward /
+nes_trainer_backward); tensors cross the boundary as CPU arrays.  For tests:
+it lets torch build losses on rendered colours and checks that gradients
+reach the hash table and MLP weights.
+"""
+import numpy as np
+import torch
+
+
+class RenderRays(torch.autograd.Function):
+    """rgb = render(table, weights; rays) for explicit rays."""
+
+    @staticmethod
+    def forward(ctx, table, weights, trainer, ray_o, ray_d):
+        trainer.set_params(table.detach().cpu().numpy(), weights.detach().cpu().numpy())
+        rgb = trainer.forward(np.asarray(ray_o, np.float32), np.asarray(ray_d, np.float32))
+        ctx.trainer = trainer
+        return torch.from_numpy(rgb)
+
+    @staticmethod
+    def backward(ctx, grad_rgb):
+        gt, gw = ctx.trainer.backward(grad_rgb.detach().cpu().numpy().astype(np.float32))
+        return torch.from_numpy(gt), torch.from_numpy(gw), None, None, None
+
+
+def render_rays(table, weights, trainer, ray_o, ray_d):
+    return RenderRays.apply(table, weights, trainer, ray_o, ray_d)

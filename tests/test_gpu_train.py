@@ -1,0 +1,128 @@
+"""Differentiable render on the GPU: forward and gradients against the FP64
+oracle (itself pinned by finite differences), the autograd binding, Adam's
+closed-form first step and a loss that falls on the config-3 setup."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def rays(oracle, scene1):
+    t, w, o = scene1
+    ro, rd, _, _ = oracle.training_rays(0, 0, 3000)
+    ns = np.array([oracle.render_ray(t, w, o, ro[i], rd[i])[3] for i in range(len(ro))])
+    keep = np.flatnonzero(ns > 0)[:300]
+    return ro, rd, keep
+
+
+@pytest.fixture(scope="module")
+def trainer(nes, scene1):
+    t, w, o = scene1
+    with nes.Trainer(t, w, o, rays_per_batch=4096) as tr:
+        yield tr
+
+
+def test_forward_matches_oracle(oracle, scene1, trainer, rays):
+    t, w, o = scene1
+    ro, rd, _ = rays
+    rgb = trainer.forward(ro, rd)
+    ref = np.array([oracle.render_ray(t, w, o, ro[i], rd[i])[0] for i in range(len(ro))])
+    assert np.abs(rgb - ref).max() < 2e-5
+
+
+def test_backward_matches_oracle_gradient(oracle, scene1, trainer, rays):
+    t, w, o = scene1
+    ro, rd, keep = rays
+    ro, rd = ro[keep], rd[keep]
+    target = np.random.default_rng(5).uniform(0, 1, (len(ro), 3)).astype(np.float32)
+    loss, gt_ref, gw_ref = oracle.loss_grad(t, w, o, ro, rd, target)
+    rgb = trainer.forward(ro, rd)
+    d_rgb = 2.0 * (rgb - target) / (3 * len(ro))
+    gt, gw = trainer.backward(d_rgb)
+    # MLP weights: every layer within 1e-3 of the largest oracle gradient
+    for a, b in [(0, 2048), (2048, 3072), (3072, 5056), (5056, 9152), (9152, 9344)]:
+        scale = np.abs(gw_ref[a:b]).max()
+        assert np.abs(gw[a:b] - gw_ref[a:b]).max() <= 1e-3 * scale, (a, b)
+    # table: same support, values within 1e-3 of the largest gradient
+    assert np.array_equal(gt != 0, gt_ref != 0) or np.abs(gt[gt_ref == 0]).max() == 0
+    scale = np.abs(gt_ref).max()
+    assert np.abs(gt - gt_ref).max() <= 1e-3 * scale
+    cos = float(gt @ gt_ref / np.linalg.norm(gt) / np.linalg.norm(gt_ref))
+    assert cos > 0.99999
+
+
+def test_autograd_binding(nes, oracle, scene1, trainer, rays):
+    torch = pytest.importorskip("torch")
+    from nes_amd.autograd import render_rays
+    t, w, o = scene1
+    ro, rd, keep = rays
+    ro, rd = ro[keep[:64]], rd[keep[:64]]
+    table = torch.tensor(t, requires_grad=True)
+    weights = torch.tensor(w, requires_grad=True)
+    target = torch.rand(len(ro), 3, generator=torch.Generator().manual_seed(0))
+    rgb = render_rays(table, weights, trainer, ro, rd)
+    loss = ((rgb - target) ** 2).mean()
+    loss.backward()
+    _, gt_ref, gw_ref = oracle.loss_grad(t, w, o, ro, rd, target.numpy())
+    assert np.abs(weights.grad.numpy() - gw_ref).max() <= 1e-3 * np.abs(gw_ref).max()
+    assert np.abs(table.grad.numpy() - gt_ref).max() <= 1e-3 * np.abs(gt_ref).max()
+    # central differences through the GPU forward on the largest weight gradients
+    base = weights.detach().clone()
+    for i in np.argsort(-np.abs(gw_ref))[:4]:
+        h = 1e-2 * max(abs(float(base[i])), 1e-2)
+        vals = []
+        for sgn in (1, -1):
+            wp = base.clone()
+            wp[i] += sgn * h
+            with torch.no_grad():
+                vals.append(float(((render_rays(table, wp, trainer, ro, rd) - target) ** 2).mean()))
+        fd = (vals[0] - vals[1]) / (2 * h)
+        assert abs(fd - gw_ref[i]) <= 0.05 * abs(gw_ref[i]), (i, fd, gw_ref[i])
+
+
+def poses_and_targets(nes, oracle, scene1, w=640, h=480):
+    t, wt, o = scene1
+    poses = np.stack([oracle.training_pose(0, i) for i in range(16)])
+    with nes.Context(t, wt, o) as ctx:
+        outs, _ = ctx.render([nes.camera(w, h, p) for p in poses], precision=nes.PRECISION_FP32)
+    return poses, np.stack([x["rgb"] for x in outs])
+
+
+def test_batch_loss_matches_oracle_and_adam_first_step(nes, oracle, scene1):
+    t, w, o = scene1
+    poses, targets = poses_and_targets(nes, oracle, scene1)
+    ts, ws, _ = oracle.scene(4, table_init=1, occ=False)
+    n = 2048
+    with nes.Trainer(ts, ws, o, poses=poses, targets=targets, width=640, height=480,
+                     rays_per_batch=n, seed=0) as tr:
+        loss, gt, gw = tr.batch_grad(0)
+        ro, rd, pose, pix = oracle.training_rays(0, 0, n)
+        tgt = targets.reshape(16, -1, 3)[pose, pix]
+        loss_ref, _, gw_ref = oracle.loss_grad(ts, ws, o, ro, rd, tgt)
+        assert abs(loss - loss_ref) <= 1e-4 * loss_ref
+        assert np.abs(gw - gw_ref).max() <= 1e-3 * np.abs(gw_ref).max()
+        l0, st = tr.step()
+        assert abs(l0 - loss) <= 1e-5 * loss and st.step == 1
+        t1, w1 = tr.get_params()
+    # first Adam step from zero moments: p -= lr * g / (|g| + eps)
+    for p0, p1, g in ((ts, t1, gt), (ws, w1, gw)):
+        want = p0 - 1e-2 * g / (np.abs(g) + 1e-15)
+        assert np.abs(p1 - want).max() <= 1e-6
+
+
+def test_config3_training_reduces_loss(nes, oracle, scene1):
+    t, w, o = scene1
+    poses, targets = poses_and_targets(nes, oracle, scene1)
+    ts, ws, _ = oracle.scene(4, table_init=1, occ=False)
+    with nes.Trainer(ts, ws, o, poses=poses, targets=targets, width=640, height=480,
+                     rays_per_batch=1 << 18, seed=0) as tr:
+        losses, ms = [], []
+        for _ in range(40):
+            l, st = tr.step()
+            losses.append(l)
+            ms.append(st.ms)
+    print(f"loss {losses[0]:.5f} -> {losses[-1]:.5f}, {np.median(ms):.2f} ms/step, "
+          f"{st.samples} samples/batch")
+    assert np.isfinite(losses).all()
+    assert np.mean(losses[-5:]) < 0.5 * np.mean(losses[:3])
