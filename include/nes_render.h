@@ -281,6 +281,36 @@ typedef struct nes_sim_config {
 int nes_server_simulate(nes_server *server, const nes_sim_config *sim,
                         nes_server_stats *stats);
 
+/* ------------------------------------------------- neserver wire protocol */
+/* A renderer endpoint for an unmodified neserver: it accepts the TCP
+ * connections of socket_manage_thread (src/server.cpp:216-283) and answers
+ * each length-prefixed FrameRequest (8-byte host-endian length + proto3
+ * message, src/server.cpp:23-112) with a RenderedFrame carrying RGB24 and
+ * GRAY8 depth (proto/nes.proto:18-25, consumed at src/server.cpp:193-194).
+ * Camera may carry an additive float field 5 (vertical FOV); without it the
+ * server's default applies.  GRAY8 depth = clamp(floor(255 d / depth_far + 0.5)). */
+typedef struct nes_wire_server nes_wire_server;
+
+int nes_wire_decode_request(const uint8_t *msg, size_t len, float default_fov,
+                            nes_camera *cam);
+int nes_wire_encode_frame(const nes_camera *cam, const uint8_t *rgb24,
+                          const uint8_t *gray8, uint8_t *out, size_t cap,
+                          size_t *len);
+int nes_wire_server_start(nes_ctx *ctx, const char *addr, uint16_t port,
+                          int precision, float default_fov, float depth_far,
+                          nes_wire_server **out);
+int nes_wire_server_port(nes_wire_server *server);
+uint64_t nes_wire_server_served(nes_wire_server *server);
+int nes_wire_server_stop(nes_wire_server *server);
+
+/* ------------------------------------------------------------ diagnostics */
+/* Detailed text of the last error (shared by all threads). */
+const char *nes_last_error(void);
+/* The vertical chroma filter of an RGB24 WxH -> YUV420P WxH conversion
+ * (libswscale initFilter, bicubic, 12-bit taps): coef is h/2 x taps. */
+int nes_sws_vchroma_filter(int h, int16_t *coef, int32_t *pos, int max_taps,
+                           int *taps);
+
 /* ------------------------------------------------------ device memory */
 /* Minimal device-memory helpers so callers (tests, benchmarks) need no other
  * GPU runtime. */

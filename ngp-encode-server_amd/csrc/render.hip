@@ -324,21 +324,31 @@ __global__ __launch_bounds__(256) void composite(
   if (!a.done && cnt) {
     const uint32_t off = offsets[slot];
     const float ts = load_ray(rays, slot).t_start;
-    for (uint32_t j = 0; j < cnt; ++j) {
-      const float4 s = sout[off + j];
-      const float t = step_t((int)(samples[off + j] & 127u), ts);
-      const float sd = s.x * kDelta;
-      const float alpha = -expm1f(-sd);
-      const float w = a.T * alpha;
-      a.r = fmaf(w, s.y, a.r);
-      a.g = fmaf(w, s.z, a.g);
-      a.b = fmaf(w, s.w, a.b);
-      a.depth = fmaf(w, t, a.depth);
-      a.T *= expf(-sd);
-      a.n_samples++;
-      if (a.T < kTStop) {
-        a.done = 1;
-        break;
+    // samples are fetched eight at a time (independent loads), composited in order
+    for (uint32_t j0 = 0; j0 < cnt && !a.done; j0 += 8) {
+      float4 sb[8];
+      uint32_t kb[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const uint32_t j = min(j0 + (uint32_t)u, cnt - 1);
+        sb[u] = sout[off + j];
+        kb[u] = samples[off + j];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        if (j0 + (uint32_t)u >= cnt || a.done) break;
+        const float4 s = sb[u];
+        const float t = step_t((int)(kb[u] & 127u), ts);
+        const float sd = s.x * kDelta;
+        const float alpha = -expm1f(-sd);
+        const float w = a.T * alpha;
+        a.r = fmaf(w, s.y, a.r);
+        a.g = fmaf(w, s.z, a.g);
+        a.b = fmaf(w, s.w, a.b);
+        a.depth = fmaf(w, t, a.depth);
+        a.T *= expf(-sd);
+        a.n_samples++;
+        if (a.T < kTStop) a.done = 1;
       }
     }
     if (!last) acc[slot] = a;

@@ -145,6 +145,14 @@ _sig("nes_trainer_set_params", C.c_int, C.c_void_p, _P(C.c_float), _P(C.c_float)
 _sig("nes_trainer_forward", C.c_int, C.c_void_p, _P(C.c_float), _P(C.c_float), C.c_uint32,
      _P(C.c_float))
 _sig("nes_trainer_backward", C.c_int, C.c_void_p, _P(C.c_float), _P(C.c_float), _P(C.c_float))
+_sig("nes_wire_decode_request", C.c_int, C.c_char_p, C.c_size_t, C.c_float, _P(Camera))
+_sig("nes_wire_encode_frame", C.c_int, _P(Camera), _P(C.c_uint8), _P(C.c_uint8), C.c_char_p,
+     C.c_size_t, _P(C.c_size_t))
+_sig("nes_wire_server_start", C.c_int, C.c_void_p, C.c_char_p, C.c_uint16, C.c_int, C.c_float,
+     C.c_float, _P(C.c_void_p))
+_sig("nes_wire_server_port", C.c_int, C.c_void_p)
+_sig("nes_wire_server_served", C.c_uint64, C.c_void_p)
+_sig("nes_wire_server_stop", C.c_int, C.c_void_p)
 _sig("nes_server_create", C.c_int, C.c_void_p, _P(ServerConfig), _P(C.c_void_p))
 _sig("nes_server_destroy", C.c_int, C.c_void_p)
 _sig("nes_server_submit", C.c_int, C.c_void_p, C.c_uint32, _P(Camera), C.c_int, _P(C.c_uint64))
@@ -513,6 +521,53 @@ class Trainer:
     def close(self):
         if self.handle:
             check(lib().nes_trainer_destroy(self.handle), "nes_trainer_destroy")
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+# --------------------------------------------------------- wire protocol
+
+def wire_decode_request(msg, default_fov=60.0):
+    cam = Camera()
+    check(lib().nes_wire_decode_request(msg, len(msg), default_fov, C.byref(cam)),
+          "nes_wire_decode_request")
+    return cam
+
+
+def wire_encode_frame(cam, rgb24=None, gray8=None):
+    r = None if rgb24 is None else np.ascontiguousarray(rgb24, np.uint8)
+    g = None if gray8 is None else np.ascontiguousarray(gray8, np.uint8)
+    n = C.c_size_t()
+    check(lib().nes_wire_encode_frame(C.byref(cam), _ptr(r, C.c_uint8), _ptr(g, C.c_uint8), None, 0,
+                                      C.byref(n)), "nes_wire_encode_frame")
+    buf = C.create_string_buffer(n.value)
+    check(lib().nes_wire_encode_frame(C.byref(cam), _ptr(r, C.c_uint8), _ptr(g, C.c_uint8), buf,
+                                      n.value, C.byref(n)), "nes_wire_encode_frame")
+    return buf.raw[: n.value]
+
+
+class WireServer:
+    """neserver-compatible renderer endpoint (length-prefixed nes.proto over TCP)."""
+
+    def __init__(self, ctx, port=0, addr="127.0.0.1", precision=PRECISION_MIXED, default_fov=60.0,
+                 depth_far=4.0):
+        h = C.c_void_p()
+        check(lib().nes_wire_server_start(ctx.handle, addr.encode(), port, precision, default_fov,
+                                          depth_far, C.byref(h)), "nes_wire_server_start")
+        self.handle = h
+        self.port = lib().nes_wire_server_port(h)
+
+    def served(self):
+        return lib().nes_wire_server_served(self.handle)
+
+    def close(self):
+        if self.handle:
+            check(lib().nes_wire_server_stop(self.handle), "nes_wire_server_stop")
             self.handle = None
 
     def __enter__(self):
