@@ -19,6 +19,7 @@ sys.path.insert(0, os.path.join(ROOT, "ngp-encode-server_amd"))
 
 METRIC = "Mrays/s mixed-prec vs FP32 at ≥45 dB PSNR; p99 frame latency, 8 clients"
 FLOP_PER_SAMPLE = 18688      # density 32-64-16 + colour 31-64-64-3 MLPs, 2 FLOP/MAC
+GATHER_BYTES_PER_SAMPLE = 512  # 16 levels x 8 corners x 2 FP16 features
 PEAK_FP16_TFLOPS = 2500.0    # MI355X dense FP16 MFMA
 PEAK_HBM_GBS = 8000.0
 
@@ -113,7 +114,10 @@ def frameprep_leg(N, n_frames=256, w=1280, h=720):
 
 
 def load_traffic():
-    p = os.path.join(ROOT, "profiles", "eval_mixed_stereo1080p_pmc.json")
+    """HBM bytes per launch of encode_parts<f16> on this workload, from the
+    committed rocprofv3 PMC summary (FETCH_SIZE + WRITE_SIZE, calibrated
+    against a plain device copy of known size)."""
+    p = os.path.join(ROOT, "profiles", "encode_parts_f16_stereo1080p_pmc.json")
     if os.path.exists(p):
         with open(p) as f:
             return json.load(f).get("hbm_bytes_per_launch")
@@ -130,6 +134,8 @@ def main():
     ap.add_argument("--skip-frameprep", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--skip-serve", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 5 device copies of 1 GiB (memory-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -137,6 +143,12 @@ def main():
         sys.exit(2)
 
     import nes_amd as N
+    if args.copy_calibration:
+        nbytes = 1 << 30
+        a, b = N.DeviceBuffer(nbytes), N.DeviceBuffer(nbytes)
+        ms = [N.device_copy(b, a, nbytes) for _ in range(5)]
+        print(json.dumps({"copy_bytes": nbytes, "ms": ms}))
+        return
     name, cus, _ = N.device_info()
     log(f"device {name} ({cus} CUs)")
     table, weights, occ = N.scene_synthesize(1)
@@ -151,11 +163,14 @@ def main():
         log(f"fp32: {wall_f / steps_f * 1e3:.3f} ms/step")
         samples = st_m[-1].samples
         ms_eval = float(np.mean([s.ms_eval for s in st_m]))
+        ms_enc = float(np.mean([s.ms_encode for s in st_m]))
+        ms_mlp = float(np.mean([s.ms_mlp for s in st_m]))
         ms_march = float(np.mean([s.ms_march for s in st_m]))
         ms_comp = float(np.mean([s.ms_composite for s in st_m]))
         value = rays * args.steps / wall_m / 1e6
         fp32 = rays * steps_f / wall_f / 1e6
-        achieved = FLOP_PER_SAMPLE * samples / (ms_eval * 1e-3) / 1e12
+        gathered_gbs = GATHER_BYTES_PER_SAMPLE * samples / (ms_enc * 1e-3) / 1e9
+        mlp_tflops = FLOP_PER_SAMPLE * samples / (ms_mlp * 1e-3) / 1e12
         result = {
             "metric": METRIC, "value": round(value, 3), "unit": "Mrays/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup,
@@ -168,13 +183,18 @@ def main():
                        "samples_per_ray": round(samples / rays, 4)},
             "fp32_mrays": round(fp32, 3),
             "speedup_vs_fp32": round(value / fp32, 3),
-            "stage_ms": {"march": round(ms_march, 4), "eval": round(ms_eval, 4),
-                         "composite": round(ms_comp, 4),
-                         "fp32_eval": round(float(np.mean([s.ms_eval for s in st_f])), 4)},
-            "roofline": {"kernel": "eval_mixed", "bound": "mfma",
-                         "achieved": round(achieved, 3), "peak": PEAK_FP16_TFLOPS,
-                         "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP16_TFLOPS, 5),
+            "stage_ms": {"march": round(ms_march, 4), "encode": round(ms_enc, 4),
+                         "mlp": round(ms_mlp, 4), "composite": round(ms_comp, 4),
+                         "fp32_encode": round(float(np.mean([s.ms_encode for s in st_f])), 4),
+                         "fp32_mlp": round(float(np.mean([s.ms_mlp for s in st_f])), 4)},
+            # dominant kernel: the FP16 hash-grid encode (512 B gathered per sample)
+            "roofline": {"kernel": "encode_parts<f16>", "bound": "hbm",
+                         "achieved": round(gathered_gbs, 1), "peak": PEAK_HBM_GBS,
+                         "unit": "GB/s", "frac": round(gathered_gbs / PEAK_HBM_GBS, 4),
                          "traffic": load_traffic()},
+            "mlp_roofline": {"kernel": "mlp_mixed", "bound": "mfma",
+                             "achieved": round(mlp_tflops, 2), "peak": PEAK_FP16_TFLOPS,
+                             "unit": "TFLOP/s", "frac": round(mlp_tflops / PEAK_FP16_TFLOPS, 5)},
             "psnr_db": None, "p99_frame_latency_ms": None, "cpu_baseline": None,
         }
         if not args.skip_frameprep:

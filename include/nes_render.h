@@ -108,6 +108,8 @@ typedef struct nes_render_stats {
   double ms_eval;           /* device time of the network kernel */
   double ms_march;
   double ms_composite;
+  double ms_encode;         /* hash-grid encode kernel (part of ms_eval) */
+  double ms_mlp;            /* MLP kernel (part of ms_eval) */
 } nes_render_stats;
 
 /* Synchronous render of n views (e.g. a stereo pair) in one launch sequence:
@@ -318,6 +320,9 @@ int nes_device_alloc(size_t bytes, void **ptr);
 int nes_device_free(void *ptr);
 int nes_memcpy_h2d(void *dst, const void *src, size_t bytes);
 int nes_memcpy_d2h(void *dst, const void *src, size_t bytes);
+/* Device-to-device copy (a plain copy of known size, used to calibrate
+ * memory counters); elapsed_ms optional. */
+int nes_device_copy(void *dst, const void *src, size_t bytes, float *elapsed_ms);
 int nes_device_info(char *name, size_t name_len, int *cu_count,
                     uint64_t *hbm_bytes);
 

@@ -123,6 +123,22 @@ int nes_memcpy_d2h(void *dst, const void *src, size_t bytes) {
   return NES_OK;
 }
 
+int nes_device_copy(void *dst, const void *src, size_t bytes, float *elapsed_ms) {
+  NES_TRY(nes::ensure_device());
+  hipStream_t s = nes::library_stream();
+  hipEvent_t e0, e1;
+  NES_HIP_TRY(hipEventCreate(&e0));
+  NES_HIP_TRY(hipEventCreate(&e1));
+  NES_HIP_TRY(hipEventRecord(e0, s));
+  NES_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s));
+  NES_HIP_TRY(hipEventRecord(e1, s));
+  NES_HIP_TRY(hipEventSynchronize(e1));
+  if (elapsed_ms) hipEventElapsedTime(elapsed_ms, e0, e1);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  return NES_OK;
+}
+
 int nes_device_info(char *name, size_t name_len, int *cu_count,
                     uint64_t *hbm_bytes) {
   NES_TRY(nes::ensure_device());

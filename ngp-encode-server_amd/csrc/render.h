@@ -77,7 +77,7 @@ int render_prepare_capacity(RenderWork &w, int precision);
 int render_reserve_samples(RenderWork &w, int precision, size_t capacity);
 int render_launch_pass(const SceneDev &scene, RenderWork &w, int precision, int k0, int k1,
                        bool first, bool last, hipStream_t stream, hipEvent_t ev_eval_begin,
-                       hipEvent_t ev_eval_end);
+                       hipEvent_t ev_eval_end, hipEvent_t ev_encode_end);
 
 // Single-stage launchers (training step): march_emit over prepared rays,
 // FP32 encode_parts, composite.

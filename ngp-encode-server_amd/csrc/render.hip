@@ -537,7 +537,7 @@ int render_reserve_samples(RenderWork &w, int precision, size_t capacity) {
 
 int render_launch_pass(const SceneDev &scene, RenderWork &w, int precision, int k0, int k1,
                        bool first, bool last, hipStream_t stream, hipEvent_t ev_eval_begin,
-                       hipEvent_t ev_eval_end) {
+                       hipEvent_t ev_eval_end, hipEvent_t ev_encode_end) {
   const unsigned blocks = (unsigned)((w.n_slots + 255) / 256);
   uint32_t *cursor = w.total.as<uint32_t>();
   const uint32_t cap = w.capacity;
@@ -556,6 +556,7 @@ int render_launch_pass(const SceneDev &scene, RenderWork &w, int precision, int 
                                                            scene.table_f32, w.feat.as<float4>(),
                                                            bpp);
     NES_HIP_TRY(hipGetLastError());
+    if (ev_encode_end) NES_HIP_TRY(hipEventRecord(ev_encode_end, stream));
     mlp_fp32<<<256 * 4, 256, 0, stream>>>(w.sample_pos.as<float4>(), w.feat.as<float4>(), cursor,
                                           cap, w.rays.as<RayData>(), scene.weights_f32,
                                           w.sample_out.as<float4>());
@@ -564,6 +565,7 @@ int render_launch_pass(const SceneDev &scene, RenderWork &w, int precision, int 
                                                               cap, scene.table_f16,
                                                               w.feat.as<uint2>(), bpp);
     NES_HIP_TRY(hipGetLastError());
+    if (ev_encode_end) NES_HIP_TRY(hipEventRecord(ev_encode_end, stream));
     mlp_mixed<<<256 * 8, 256, 0, stream>>>(w.sample_pos.as<float4>(), w.feat.as<uint2>(), cursor,
                                            cap, w.rays.as<RayData>(),
                                            reinterpret_cast<const half8 *>(scene.wfrag),
@@ -632,7 +634,7 @@ int render_enqueue(const SceneDev &scene, RenderWork &w, int precision,
   const int spp = steps_per_pass == 0 ? kMaxSteps : std::min(steps_per_pass, kMaxSteps);
   NES_TRY(render_setup(w, views, n_views, stream));
   NES_TRY(render_prepare_capacity(w, precision));
-  double ms_march = 0, ms_eval = 0, ms_comp = 0;
+  double ms_march = 0, ms_eval = 0, ms_comp = 0, ms_enc = 0;
   uint64_t n_samples = 0;
   NES_HIP_TRY(hipEventRecord(w.ev[0], stream));
   const int passes = (kMaxSteps + spp - 1) / spp;
@@ -641,7 +643,7 @@ int render_enqueue(const SceneDev &scene, RenderWork &w, int precision,
     for (int attempt = 0;; ++attempt) {
       NES_HIP_TRY(hipEventRecord(w.ev[1], stream));
       NES_TRY(render_launch_pass(scene, w, precision, k0, k1, p == 0, p == passes - 1, stream,
-                                 w.ev[2], w.ev[3]));
+                                 w.ev[2], w.ev[3], w.ev[5]));
       NES_HIP_TRY(hipEventRecord(w.ev[4], stream));
       NES_HIP_TRY(hipEventSynchronize(w.ev[4]));
       const uint32_t n = w.total_host[0];
@@ -655,17 +657,21 @@ int render_enqueue(const SceneDev &scene, RenderWork &w, int precision,
       }
       NES_TRY(render_reserve_samples(w, precision, (size_t)n + n / 8 + 1024));
     }
-    float a = 0, b = 0, c = 0;
+    float a = 0, b = 0, c = 0, e = 0;
     hipEventElapsedTime(&a, w.ev[1], w.ev[2]);
     hipEventElapsedTime(&b, w.ev[2], w.ev[3]);
     hipEventElapsedTime(&c, w.ev[3], w.ev[4]);
+    hipEventElapsedTime(&e, w.ev[2], w.ev[5]);
     ms_march += a;
     ms_eval += b;
     ms_comp += c;
+    ms_enc += e;
   }
   float t = 0;
   hipEventElapsedTime(&t, w.ev[0], w.ev[4]);
   if (stats) {
+    stats->ms_encode = ms_enc;
+    stats->ms_mlp = ms_eval - ms_enc;
     stats->rays = w.out_px;
     stats->samples = n_samples;
     stats->ms_total = t;

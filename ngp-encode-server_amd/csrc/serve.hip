@@ -233,7 +233,7 @@ static void finish_batch(nes_server *s, Slot &sl, int status) {
 static int launch_batch(nes_server *s, Slot &sl) {
   const SceneDev &scene = s->ctx->scene;
   NES_TRY(render_launch_pass(scene, sl.work, s->cfg.precision, 0, kMaxSteps, true, true,
-                             sl.stream, nullptr, nullptr));
+                             sl.stream, nullptr, nullptr, nullptr));
   NES_TRY(launch_output(s, sl));
   NES_HIP_TRY(hipEventRecord(sl.ev_done, sl.stream));
   return NES_OK;

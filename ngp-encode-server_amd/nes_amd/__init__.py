@@ -56,7 +56,8 @@ class ViewOut(C.Structure):
 
 class RenderStats(C.Structure):
     _fields_ = [("rays", C.c_uint64), ("samples", C.c_uint64), ("ms_total", C.c_double),
-                ("ms_eval", C.c_double), ("ms_march", C.c_double), ("ms_composite", C.c_double)]
+                ("ms_eval", C.c_double), ("ms_march", C.c_double), ("ms_composite", C.c_double),
+                ("ms_encode", C.c_double), ("ms_mlp", C.c_double)]
 
 
 class ServerConfig(C.Structure):
@@ -165,6 +166,7 @@ _sig("nes_device_alloc", C.c_int, C.c_size_t, _P(C.c_void_p))
 _sig("nes_device_free", C.c_int, C.c_void_p)
 _sig("nes_memcpy_h2d", C.c_int, C.c_void_p, C.c_void_p, C.c_size_t)
 _sig("nes_memcpy_d2h", C.c_int, C.c_void_p, C.c_void_p, C.c_size_t)
+_sig("nes_device_copy", C.c_int, C.c_void_p, C.c_void_p, C.c_size_t, _P(C.c_float))
 _sig("nes_device_info", C.c_int, C.c_char_p, C.c_size_t, _P(C.c_int), _P(C.c_uint64))
 
 _lib = None
@@ -371,6 +373,13 @@ def frameprep_device(rgb_a, depth_a, rgb_b, depth_b, w, h, n_frames, y, u, v, mo
     check(lib().nes_frameprep_device(vp(rgb_a), vp(depth_a), vp(rgb_b), vp(depth_b), w, h,
                                      n_frames, vp(y), w, vp(u), w // 2, vp(v), w // 2, mode,
                                      C.byref(ms)), "nes_frameprep_device")
+    return ms.value
+
+
+def device_copy(dst, src, nbytes):
+    """dst/src: DeviceBuffer; returns the copy time in ms."""
+    ms = C.c_float()
+    check(lib().nes_device_copy(dst.ptr, src.ptr, nbytes, C.byref(ms)), "nes_device_copy")
     return ms.value
 
 

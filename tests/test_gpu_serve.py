@@ -61,3 +61,21 @@ def test_poisson_clients_short_run(nes, gpu_ctx):
     assert st.completed > 0 and st.failed == 0
     assert 0 < st.p50_ms <= st.p99_ms <= st.max_ms
     assert st.mrays_per_s > 0
+
+
+def test_hbm_budget_decides_batch_slots(nes, gpu_ctx):
+    """Admission by memory: the number of concurrent batch slots is what the
+    budget affords, and a budget below one slot is refused."""
+    small_rays = 1 << 20
+    with nes.Server(gpu_ctx, max_batch_rays=small_rays, max_inflight=8) as srv:
+        assert srv.stats().slots == 8
+    # one slot of 2^20 ray slots needs about (64 + 31) B + 16 x 20 B per ray
+    per_slot = small_rays * (32 + 32 + 8 + 21 + 2 + 16 * 20) + (1 << 20)
+    with nes.Server(gpu_ctx, max_batch_rays=small_rays, max_inflight=8,
+                    hbm_budget=2 * per_slot + 1) as srv:
+        assert srv.stats().slots == 2
+        t = srv.submit(0, stereo(nes, 320, 240, 40.0))
+        assert srv.wait(t)[0] == 0
+    with pytest.raises(nes.NesError) as e:
+        nes.Server(gpu_ctx, max_batch_rays=small_rays, hbm_budget=per_slot // 2)
+    assert e.value.code == -12
