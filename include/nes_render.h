@@ -158,6 +158,71 @@ int nes_gray8_to_yuv420p(const uint8_t *gray, int w, int h, int g_stride,
                          uint8_t *y, int y_stride, uint8_t *u, int u_stride,
                          uint8_t *v, int v_stride, int sws_mode);
 
+/* ------------------------------------------------------------ text overlay */
+/* The frame-prep driver stamps four strings into the scene frame before the
+ * YUV conversion (process_frame_thread, src/encode.cpp:57-97) with
+ * RenderTextContext::render_string_to_frame (src/base/video/
+ * render_text.cc:35-111): every non-zero pixel of a glyph bitmap turns the
+ * RGB24 pixel under it white.  Glyph rasterisation stays with the caller's
+ * FreeType (one FT_Load_Char(face, c, FT_LOAD_RENDER) per character, done
+ * once per font); this library takes the resulting glyph table and does the
+ * layout and the stamping on the device. */
+typedef struct nes_glyph {
+  int32_t left;      /* slot->bitmap_left */
+  int32_t top;       /* slot->bitmap_top */
+  uint32_t width;    /* slot->bitmap.width (also the row pitch used) */
+  uint32_t rows;     /* slot->bitmap.rows */
+  int32_t advance;   /* slot->advance.x >> 6 */
+  uint32_t offset;   /* byte offset of the rows x width bitmap in `bitmaps` */
+} nes_glyph;
+
+#define NES_FONT_GLYPHS 256 /* indexed by the unsigned byte of the character */
+
+/* RenderTextContext::RenderPosition (include/base/video/render_text.h:17-23) */
+#define NES_TEXT_LEFT_TOP 0
+#define NES_TEXT_LEFT_BOTTOM 1
+#define NES_TEXT_RIGHT_TOP 2
+#define NES_TEXT_RIGHT_BOTTOM 3
+#define NES_TEXT_CENTER 4
+
+typedef struct nes_text {
+  int32_t position;  /* NES_TEXT_* */
+  const char *text;  /* NUL-terminated; '\n' starts a line 20 px lower */
+} nes_text;
+
+/* The four strings of process_frame_thread, as bits of overlay_flags. */
+#define NES_OVERLAY_MATRIX 1u    /* camera matrix, centre */
+#define NES_OVERLAY_INDEX 2u     /* "index=<n>", left bottom */
+#define NES_OVERLAY_TIMESTAMP 4u /* local "%T.mmm", left top */
+#define NES_OVERLAY_DIRECTION 8u /* "direction=left|right", right top */
+#define NES_OVERLAY_ALL 15u
+
+typedef struct nes_font nes_font;
+/* Replaces RenderTextContext's constructor (render_text.cc:10-33): glyphs
+ * holds NES_FONT_GLYPHS entries; each bitmap must lie inside bitmaps. */
+int nes_font_create(const nes_glyph *glyphs, const uint8_t *bitmaps,
+                    size_t bitmap_bytes, nes_font **out);
+int nes_font_destroy(nes_font *font);
+/* render_string_to_frame for n texts, in place on a device RGB24 frame. */
+int nes_overlay_device(nes_font *font, const nes_text *texts, int n_texts,
+                       uint8_t *rgb_dev, int w, int h, int rgb_stride);
+/* The strings process_frame_thread draws for a frame (src/encode.cpp:57-97),
+ * NUL-separated into buf in the order matrix, index, timestamp, direction
+ * (only those selected by flags); timestamp may be NULL for the current
+ * local time.  Returns the bytes used, or a negative code. */
+int nes_overlay_texts(const nes_camera *cam, const char *timestamp,
+                      uint32_t flags, char *buf, size_t buf_len);
+/* process_frame_thread's per-frame work in one call: the optional
+ * nearer-depth composite, the selected strings of `cam` stamped into the
+ * scene, then RGB24 -> YUV420P (host buffers, as nes_frameprep). */
+int nes_frameprep_overlay(nes_font *font, const nes_camera *cam,
+                          const char *timestamp, uint32_t flags,
+                          const uint8_t *rgb_a, const float *depth_a,
+                          const uint8_t *rgb_b, const float *depth_b, int w,
+                          int h, int rgb_stride, uint8_t *y, int y_stride,
+                          uint8_t *u, int u_stride, uint8_t *v, int v_stride,
+                          int sws_mode);
+
 /* -------------------------------------------------------------- training */
 /* Differentiable render step for the same scene representation: L2
  * photometric loss, hand-written backward through compositing, the MLPs
@@ -229,6 +294,8 @@ typedef struct nes_server_config {
   uint32_t queue_capacity;   /* waiting requests, 0 = 256 */
   int precision;             /* NES_PRECISION_* */
   int sws_mode;              /* NES_SWS_* */
+  nes_font *font;            /* text overlay font, or NULL for none */
+  uint32_t overlay_flags;    /* NES_OVERLAY_* drawn on every view */
 } nes_server_config;
 
 typedef struct nes_frame_result {

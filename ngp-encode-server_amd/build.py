@@ -10,7 +10,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "libnes_render.so")
-SOURCES = ["common.hip", "scene.hip", "frameprep.hip", "render.hip", "train.hip", "serve.hip", "wire.hip"]
+SOURCES = ["common.hip", "scene.hip", "frameprep.hip", "overlay.hip", "render.hip", "train.hip",
+           "serve.hip", "wire.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",

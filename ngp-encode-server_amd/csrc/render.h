@@ -94,6 +94,24 @@ int render_fetch(RenderWork &work, int view, const nes_view_out *out,
                  hipStream_t stream);
 void render_work_free(RenderWork &w);
 
+// Text overlay: one drawn glyph of one string on one RGB24 frame.
+struct GlyphPlacement {  // 32 bytes
+  uint64_t base;         // byte offset of the frame in the RGB24 buffer
+  int32_t x0, y0;        // top-left corner of the glyph bitmap
+  int32_t w, h, stride;  // frame size and RGB24 row pitch in bytes
+  uint32_t glyph;
+};
+// Placements of the strings process_frame_thread draws on the frame of cam
+// (its width and height are the frame's); timestamp NULL = local time now.
+void overlay_frame_layout(const nes_font *font, const nes_camera &cam, const char *timestamp,
+                          uint32_t flags, int stride, uint64_t base,
+                          std::vector<GlyphPlacement> &out);
+// Upload the placements through `staging` and stamp them on `rgb`; `place`
+// stays alive and `staging` untouched until the stream has passed the launch.
+int overlay_launch(const nes_font *font, DevBuf &staging,
+                   const std::vector<GlyphPlacement> &place, uint8_t *rgb, hipStream_t stream);
+std::string overlay_now_text();
+
 }  // namespace nes
 
 struct nes_ctx {

@@ -107,6 +107,8 @@ struct Slot {
   hipStream_t stream = nullptr;
   RenderWork work;
   DevBuf yuv;
+  DevBuf ovl_place;                    // text overlay placements (device)
+  std::vector<GlyphPlacement> ovl_host;  // ... and their host copy
   hipEvent_t ev_done = nullptr;
   std::vector<std::shared_ptr<Request>> reqs;
   std::vector<nes_camera> views;
@@ -155,6 +157,18 @@ static int launch_output(nes_server *s, Slot &sl) {
   size_t yuv_total = 0;
   for (const nes_camera &c : sl.views) yuv_total += yuv_bytes(c);
   NES_TRY(sl.yuv.reserve(yuv_total + 256));
+  if (s->cfg.font && s->cfg.overlay_flags) {
+    // the strings of every view, stamped by one launch before frame prep
+    sl.ovl_host.clear();
+    const std::string now = (s->cfg.overlay_flags & NES_OVERLAY_TIMESTAMP)
+                                ? overlay_now_text() : std::string();
+    for (size_t i = 0; i < sl.views.size(); ++i)
+      overlay_frame_layout(s->cfg.font, sl.views[i], now.c_str(), s->cfg.overlay_flags,
+                           3 * (int)sl.views[i].width, L.rgb24 + 3 * (size_t)w.host_views[i].px_base,
+                           sl.ovl_host);
+    NES_TRY(overlay_launch(s->cfg.font, sl.ovl_place, sl.ovl_host, w.out.as<uint8_t>(),
+                           sl.stream));
+  }
   size_t off = 0;
   size_t vi = 0;
   for (auto &rq : sl.reqs) {
@@ -313,6 +327,7 @@ int nes_server_create(nes_ctx *ctx, const nes_server_config *cfg, nes_server **o
   if (s->cfg.max_batch_rays > kMaxSlots) return NES_ERANGE;
   if (s->cfg.precision != NES_PRECISION_FP32 && s->cfg.precision != NES_PRECISION_MIXED) return NES_EINVAL;
   if (s->cfg.sws_mode != NES_SWS_X86 && s->cfg.sws_mode != NES_SWS_C) return NES_EINVAL;
+  if (s->cfg.overlay_flags & ~NES_OVERLAY_ALL) return NES_EINVAL;
   // admission by memory: as many batch slots as the budget holds
   const size_t per_slot = slot_bytes(s->cfg.max_batch_rays, 16);
   const uint64_t afford = s->cfg.hbm_budget_bytes / per_slot;
@@ -351,6 +366,7 @@ int nes_server_destroy(nes_server *s) {
     hipStreamSynchronize(sl->stream);
     render_work_free(sl->work);
     sl->yuv.release();
+    sl->ovl_place.release();
     hipEventDestroy(sl->ev_done);
     hipStreamDestroy(sl->stream);
   }

@@ -63,7 +63,17 @@ class RenderStats(C.Structure):
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_batch_rays", C.c_uint32),
                 ("max_inflight", C.c_uint32), ("queue_capacity", C.c_uint32),
-                ("precision", C.c_int), ("sws_mode", C.c_int)]
+                ("precision", C.c_int), ("sws_mode", C.c_int), ("font", C.c_void_p),
+                ("overlay_flags", C.c_uint32)]
+
+
+class Glyph(C.Structure):
+    _fields_ = [("left", C.c_int32), ("top", C.c_int32), ("width", C.c_uint32),
+                ("rows", C.c_uint32), ("advance", C.c_int32), ("offset", C.c_uint32)]
+
+
+class Text(C.Structure):
+    _fields_ = [("position", C.c_int32), ("text", C.c_char_p)]
 
 
 class FrameResult(C.Structure):
@@ -154,6 +164,14 @@ _sig("nes_wire_server_start", C.c_int, C.c_void_p, C.c_char_p, C.c_uint16, C.c_i
 _sig("nes_wire_server_port", C.c_int, C.c_void_p)
 _sig("nes_wire_server_served", C.c_uint64, C.c_void_p)
 _sig("nes_wire_server_stop", C.c_int, C.c_void_p)
+_sig("nes_font_create", C.c_int, _P(Glyph), _P(C.c_uint8), C.c_size_t, _P(C.c_void_p))
+_sig("nes_font_destroy", C.c_int, C.c_void_p)
+_sig("nes_overlay_device", C.c_int, C.c_void_p, _P(Text), C.c_int, C.c_void_p, C.c_int, C.c_int,
+     C.c_int)
+_sig("nes_overlay_texts", C.c_int, _P(Camera), C.c_char_p, C.c_uint32, C.c_char_p, C.c_size_t)
+_sig("nes_frameprep_overlay", C.c_int, C.c_void_p, _P(Camera), C.c_char_p, C.c_uint32,
+     _P(C.c_uint8), _P(C.c_float), _P(C.c_uint8), _P(C.c_float), C.c_int, C.c_int, C.c_int,
+     _P(C.c_uint8), C.c_int, _P(C.c_uint8), C.c_int, _P(C.c_uint8), C.c_int, C.c_int)
 _sig("nes_server_create", C.c_int, C.c_void_p, _P(ServerConfig), _P(C.c_void_p))
 _sig("nes_server_destroy", C.c_int, C.c_void_p)
 _sig("nes_server_submit", C.c_int, C.c_void_p, C.c_uint32, _P(Camera), C.c_int, _P(C.c_uint64))
@@ -318,6 +336,81 @@ def frameprep(rgb_a, depth_a=None, rgb_b=None, depth_b=None, mode=SWS_X86):
     return y, u, v
 
 
+# ------------------------------------------------------------ text overlay
+
+TEXT_LEFT_TOP, TEXT_LEFT_BOTTOM, TEXT_RIGHT_TOP, TEXT_RIGHT_BOTTOM, TEXT_CENTER = range(5)
+OVERLAY_MATRIX, OVERLAY_INDEX, OVERLAY_TIMESTAMP, OVERLAY_DIRECTION = 1, 2, 4, 8
+OVERLAY_ALL = 15
+FONT_GLYPHS = 256
+
+
+class Font:
+    """Glyph table uploaded once (RenderTextContext's FreeType face, rasterised
+    by the caller).  glyphs: FONT_GLYPHS (left, top, width, rows, advance,
+    offset) tuples; bitmaps: the uint8 atlas the offsets point into."""
+
+    def __init__(self, glyphs, bitmaps):
+        if len(glyphs) != FONT_GLYPHS:
+            raise ValueError("need %d glyphs" % FONT_GLYPHS)
+        self.glyphs = (Glyph * FONT_GLYPHS)(*[Glyph(*g) for g in glyphs])
+        self.bitmaps = np.ascontiguousarray(bitmaps, np.uint8).reshape(-1)
+        h = C.c_void_p()
+        check(lib().nes_font_create(self.glyphs, _ptr(self.bitmaps, C.c_uint8),
+                                    self.bitmaps.size, C.byref(h)), "nes_font_create")
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            check(lib().nes_font_destroy(self.handle), "nes_font_destroy")
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def overlay_device(font, texts, rgb_dev, w, h, stride=None):
+    """render_string_to_frame for [(position, str)] in place on a device
+    RGB24 frame (DeviceBuffer or raw pointer)."""
+    arr = (Text * max(len(texts), 1))(*[Text(p, t.encode()) for p, t in texts])
+    ptr = rgb_dev.ptr if isinstance(rgb_dev, DeviceBuffer) else rgb_dev
+    check(lib().nes_overlay_device(font.handle, arr, len(texts), ptr, w, h,
+                                   3 * w if stride is None else stride), "nes_overlay_device")
+
+
+def overlay_texts(cam, timestamp=None, flags=OVERLAY_ALL):
+    """The strings process_frame_thread draws for cam, in the order matrix,
+    index, timestamp, direction (those selected by flags)."""
+    buf = C.create_string_buffer(4096)
+    n = lib().nes_overlay_texts(C.byref(cam), None if timestamp is None else timestamp.encode(),
+                                flags, buf, len(buf))
+    check(n if n < 0 else NES_OK, "nes_overlay_texts")
+    return [t.decode() for t in buf.raw[:n].split(b"\0")[:-1]]
+
+
+def frameprep_overlay(font, cam, rgb_a, depth_a=None, rgb_b=None, depth_b=None, flags=OVERLAY_ALL,
+                      timestamp=None, mode=SWS_X86):
+    """Composite, text overlay of cam's strings, RGB24 -> YUV420P; (y, u, v)."""
+    h, w, _ = rgb_a.shape
+    rgb_a = np.ascontiguousarray(rgb_a, np.uint8)
+    if depth_a is not None:
+        depth_a = np.ascontiguousarray(depth_a, np.float32)
+    if rgb_b is not None:
+        rgb_b = np.ascontiguousarray(rgb_b, np.uint8)
+        depth_b = np.ascontiguousarray(depth_b, np.float32)
+    y = np.empty((h, w), np.uint8)
+    u = np.empty((h // 2, w // 2), np.uint8)
+    v = np.empty((h // 2, w // 2), np.uint8)
+    check(lib().nes_frameprep_overlay(
+        font.handle, C.byref(cam), None if timestamp is None else timestamp.encode(), flags,
+        _ptr(rgb_a, C.c_uint8), _ptr(depth_a, C.c_float), _ptr(rgb_b, C.c_uint8),
+        _ptr(depth_b, C.c_float), w, h, 3 * w, _ptr(y, C.c_uint8), w, _ptr(u, C.c_uint8), w // 2,
+        _ptr(v, C.c_uint8), w // 2, mode), "nes_frameprep_overlay")
+    return y, u, v
+
+
 def gray8_to_yuv420p(g, mode=SWS_X86):
     h, w = g.shape
     g = np.ascontiguousarray(g, np.uint8)
@@ -400,10 +493,11 @@ class Server:
     """Continuous-batching render server over a Context (nes_server)."""
 
     def __init__(self, ctx, hbm_budget=0, max_batch_rays=0, max_inflight=0, queue_capacity=0,
-                 precision=PRECISION_MIXED, sws_mode=SWS_X86):
+                 precision=PRECISION_MIXED, sws_mode=SWS_X86, font=None, overlay_flags=0):
         self.ctx = ctx
+        self.font = font  # must outlive the server
         cfg = ServerConfig(hbm_budget, max_batch_rays, max_inflight, queue_capacity, precision,
-                           sws_mode)
+                           sws_mode, font.handle if font is not None else None, overlay_flags)
         h = C.c_void_p()
         check(lib().nes_server_create(ctx.handle, C.byref(cfg), C.byref(h)), "nes_server_create")
         self.handle = h

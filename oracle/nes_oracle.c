@@ -987,3 +987,53 @@ int orc_frameprep(const uint8_t *rgb_a, const float *depth_a,
   free(tmp);
   return rc;
 }
+
+/* ------------------------------------------------------------ text overlay
+ * render_text.cc:35-111.  The pen starts per position (x_box 300, y_box 100,
+ * margin 50, :48-75) computed, as there, from the unsigned frame size and
+ * stored in a signed int; '\n' returns the pen to its starting x and moves
+ * it 20 px down (:80-84); every other byte loads its glyph (:86, indexed here
+ * by the unsigned byte) and each non-zero bitmap byte at row q, column p
+ * writes (255, 255, 255) at (pen_x + left + p, pen_y - top + q) when that
+ * lies inside the frame (:92-105); the pen then advances (:108). */
+void orc_render_string(uint8_t *surface, uint32_t width, uint32_t height,
+                       const orc_glyph *glyphs, const uint8_t *bitmaps,
+                       int position, const char *content) {
+  const int x_box = 300, y_box = 100, margin = 50;
+  int pen_x, pen_y;
+  switch (position) {
+    case 0: pen_x = margin; pen_y = margin; break;
+    case 1: pen_x = margin; pen_y = (int)(height - y_box + margin); break;
+    case 2: pen_x = (int)(width - x_box + margin); pen_y = margin; break;
+    case 3:
+      pen_x = (int)(width - x_box + margin);
+      pen_y = (int)(height - y_box + margin);
+      break;
+    case 4: pen_x = (int)(width / 2 - x_box); pen_y = (int)(height / 2 - y_box); break;
+    default: pen_x = margin; pen_y = margin; break;
+  }
+  const int orig_pen_x = pen_x;
+  for (const char *ch = content; *ch; ++ch) {
+    if (*ch == '\n') {
+      pen_x = orig_pen_x;
+      pen_y += 20;
+      continue;
+    }
+    const orc_glyph *g = &glyphs[(unsigned char)*ch];
+    const uint8_t *bm = bitmaps + g->offset;
+    const int x_first = pen_x + g->left, y_first = pen_y - g->top;
+    const int x_max = x_first + (int)g->width, y_max = y_first + (int)g->rows;
+    for (int j = y_first, q = 0; j < y_max; ++j, ++q) {
+      for (int i = x_first, p = 0; i < x_max; ++i, ++p) {
+        if (i < 0 || j < 0 || (uint32_t)i >= width || (uint32_t)j >= height) continue;
+        if (bm[(size_t)q * g->width + p]) {
+          uint8_t *px = surface + ((size_t)j * width + (size_t)i) * 3;
+          px[0] = 255;
+          px[1] = 255;
+          px[2] = 255;
+        }
+      }
+    }
+    pen_x += g->advance;
+  }
+}

@@ -153,6 +153,20 @@ int orc_frameprep(const uint8_t *rgb_a, const float *depth_a,
                   uint8_t *y, int y_stride, uint8_t *u, int u_stride,
                   uint8_t *v, int v_stride, int mode);
 
+/* Text overlay: RenderTextContext::render_string_to_frame
+ * (src/base/video/render_text.cc:35-111) on a tightly packed RGB24 frame,
+ * with each character's FreeType glyph given as a table entry (bitmap_left,
+ * bitmap_top, bitmap.width, bitmap.rows, advance.x >> 6, atlas offset). */
+typedef struct orc_glyph {
+  int32_t left, top;
+  uint32_t width, rows;
+  int32_t advance;
+  uint32_t offset;
+} orc_glyph;
+void orc_render_string(uint8_t *surface, uint32_t width, uint32_t height,
+                       const orc_glyph *glyphs, const uint8_t *bitmaps,
+                       int position, const char *content);
+
 #ifdef __cplusplus
 }
 #endif

@@ -52,6 +52,8 @@ def _declare(L):
                                          _p(C.c_uint32), _p(C.c_int32), _p(C.c_int16))
     sig = {
         "orc_mix64": (C.c_uint64, [C.c_uint64]),
+        "orc_render_string": (None, [u8p, C.c_uint32, C.c_uint32, _p(Glyph), u8p, C.c_int,
+                                     C.c_char_p]),
         "orc_rand_u64": (C.c_uint64, [C.c_uint64, C.c_uint64, C.c_uint64]),
         "orc_uniform": (C.c_double, [C.c_uint64, C.c_uint64, C.c_uint64]),
         "orc_normal": (C.c_double, [C.c_uint64, C.c_uint64, C.c_uint64]),
@@ -245,6 +247,23 @@ def composite(rgb_a, depth_a, rgb_b=None, depth_b=None):
                         ptr(depth_b, C.c_float), w, h, 3 * w, ptr(out, C.c_uint8),
                         ptr(dout, C.c_float))
     return out, dout
+
+
+class Glyph(C.Structure):
+    _fields_ = [("left", C.c_int32), ("top", C.c_int32), ("width", C.c_uint32),
+                ("rows", C.c_uint32), ("advance", C.c_int32), ("offset", C.c_uint32)]
+
+
+def render_string(rgb, glyphs, bitmaps, position, text):
+    """render_string_to_frame on an (H, W, 3) uint8 array, in place."""
+    assert rgb.dtype == np.uint8 and rgb.flags.c_contiguous and rgb.shape[2] == 3
+    h, w, _ = rgb.shape
+    tab = (Glyph * 256)(*[Glyph(*g) for g in glyphs])
+    bm = np.ascontiguousarray(bitmaps, np.uint8).reshape(-1)
+    lib().orc_render_string(rgb.ctypes.data_as(C.POINTER(C.c_uint8)), w, h, tab,
+                            bm.ctypes.data_as(C.POINTER(C.c_uint8)), position,
+                            text.encode("latin-1"))
+    return rgb
 
 
 def frameprep(rgb_a, depth_a, rgb_b=None, depth_b=None, mode=MODE_X86):
