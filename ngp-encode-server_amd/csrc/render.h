@@ -80,7 +80,8 @@ int render_launch_pass(const SceneDev &scene, RenderWork &w, int precision, int 
                        hipEvent_t ev_eval_end, hipEvent_t ev_encode_end);
 
 // Single-stage launchers (training step): march_emit over prepared rays,
-// FP32 encode_parts, composite.
+// FP32 encode_parts, composite (which always stores the final per-ray state
+// in work.acc for the backward).
 int launch_march_emit(const uint32_t *occ, RenderWork &w, int k0, int k1, hipStream_t stream);
 int launch_encode_f32(const float *table, RenderWork &w, hipStream_t stream);
 int launch_composite(RenderWork &w, bool last, hipStream_t stream);

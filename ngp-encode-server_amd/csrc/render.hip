@@ -315,7 +315,7 @@ __global__ __launch_bounds__(256) void composite(
     uint32_t n_slots, const RayData *__restrict__ rays, RayAcc *acc,
     const uint32_t *__restrict__ counts, const uint32_t *__restrict__ offsets,
     const uint32_t *__restrict__ samples, const float4 *__restrict__ sout,
-    const uint32_t *__restrict__ n_ptr, uint32_t capacity, int first, int last, OutPtrs o) {
+    const uint32_t *__restrict__ n_ptr, uint32_t capacity, int keep, int last, OutPtrs o) {
   const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
   if (slot >= n_slots) return;
   if (*n_ptr > capacity) return;  // overflowed pass: the host re-renders
@@ -351,9 +351,10 @@ __global__ __launch_bounds__(256) void composite(
         if (a.T < kTStop) a.done = 1;
       }
     }
-    if (!last) acc[slot] = a;
+    // the state carries into the next pass; after the last pass only a
+    // caller that reads it again (keep: the training backward) stores it
+    if (!last || keep) acc[slot] = a;
   }
-  (void)first;
   if (last && a.pad != 0xffffffffu) {
     const uint32_t p = a.pad;
     const float c[3] = {a.r, a.g, a.b};
@@ -576,7 +577,7 @@ int render_launch_pass(const SceneDev &scene, RenderWork &w, int precision, int 
   composite<<<blocks, 256, 0, stream>>>(w.n_slots, w.rays.as<RayData>(), w.acc.as<RayAcc>(),
                                        w.counts.as<uint32_t>(), w.offsets.as<uint32_t>(),
                                        w.samples.as<uint32_t>(), w.sample_out.as<float4>(),
-                                       cursor, cap, first ? 1 : 0, last ? 1 : 0, out_ptrs(w));
+                                       cursor, cap, 0, last ? 1 : 0, out_ptrs(w));
   NES_HIP_TRY(hipGetLastError());
   NES_HIP_TRY(hipMemcpyAsync(w.total_host, cursor, 4, hipMemcpyDeviceToHost, stream));
   return NES_OK;
@@ -611,7 +612,7 @@ int launch_composite(RenderWork &w, bool last, hipStream_t stream) {
   composite<<<blocks, 256, 0, stream>>>(w.n_slots, w.rays.as<RayData>(), w.acc.as<RayAcc>(),
                                        w.counts.as<uint32_t>(), w.offsets.as<uint32_t>(),
                                        w.samples.as<uint32_t>(), w.sample_out.as<float4>(),
-                                       w.total.as<uint32_t>(), w.capacity, 0, last ? 1 : 0,
+                                       w.total.as<uint32_t>(), w.capacity, 1, last ? 1 : 0,
                                        out_ptrs(w));
   NES_HIP_TRY(hipGetLastError());
   return NES_OK;

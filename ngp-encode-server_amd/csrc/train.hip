@@ -351,8 +351,11 @@ __global__ void train_loss(const float *rgb, const float *target, uint32_t n, fl
 
 // Per ray: walk the composited samples again and form the gradients with
 // respect to each sample's log-density and colour pre-activations.  With
-// w_i = T_i a_i and S_i = sum_{k>i} w_k c_k:
-//   dC/dsigma_i = delta (T_{i+1} c_i - S_i),   dC/dc_i = w_i
+// w_i = T_i a_i, g = dL/dC and S_i = sum_{k>i} w_k (c_k . g):
+//   dL/dsigma_i = delta (T_{i+1} (c_i . g) - S_i),   dL/dc_i = w_i g
+// The suffix sums are accumulated back to front (a forward pass parks
+// T_{i+1} (c_i . g) and w_i (c_i . g) in the output slots), so no late sample
+// takes its S_i as the difference of two nearly equal totals.
 __global__ __launch_bounds__(256) void train_dcomposite(
     uint32_t n_slots, const RayData *__restrict__ rays, const RayAcc *__restrict__ acc,
     const uint32_t *__restrict__ counts, const uint32_t *__restrict__ offsets,
@@ -363,26 +366,29 @@ __global__ __launch_bounds__(256) void train_dcomposite(
   if (!cnt) return;
   const uint32_t off = offsets[slot];
   const RayAcc a = acc[slot];
+  const uint32_t used = min(cnt, a.n_samples);
   const float g[3] = {dC[3 * a.pad + 0], dC[3 * a.pad + 1], dC[3 * a.pad + 2]};
-  const float Cg = a.r * g[0] + a.g * g[1] + a.b * g[2];
-  float T = 1.0f, Pg = 0.0f;
-  for (uint32_t k = 0; k < cnt; ++k) {
-    if (k >= a.n_samples) {
-      dout[off + k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      continue;
-    }
+  float T = 1.0f;
+  for (uint32_t k = 0; k < used; ++k) {
     const float4 s = sout[off + k];
     const float sd = s.x * kDelta;
-    const float alpha = -expm1f(-sd);
-    const float w = T * alpha;
+    const float w = T * -expm1f(-sd);
     const float Tn = T * expf(-sd);
     const float cg = s.y * g[0] + s.z * g[1] + s.w * g[2];
-    Pg += w * cg;
-    const float dsigma = kDelta * (Tn * cg - (Cg - Pg));
-    dout[off + k] = make_float4(dsigma * s.x, w * g[0] * s.y * (1.0f - s.y),
-                                w * g[1] * s.z * (1.0f - s.z), w * g[2] * s.w * (1.0f - s.w));
+    dout[off + k] = make_float4(Tn * cg, w * cg, w, 0.0f);
     T = Tn;
   }
+  float S = 0.0f;
+  for (uint32_t k = used; k-- > 0;) {
+    const float4 p = dout[off + k];
+    const float4 s = sout[off + k];
+    const float dsigma = kDelta * (p.x - S);
+    S += p.y;
+    const float w = p.z;
+    dout[off + k] = make_float4(dsigma * s.x, w * g[0] * s.y * (1.0f - s.y),
+                                w * g[1] * s.z * (1.0f - s.z), w * g[2] * s.w * (1.0f - s.w));
+  }
+  for (uint32_t k = used; k < cnt; ++k) dout[off + k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   (void)rays;
 }
 

@@ -442,6 +442,34 @@ void orc_render_ray(const float *table, const float *weights,
   render_ray_p(&p, occ, o, d, res);
 }
 
+/* Smallest |pre-activation| of the three ReLU layers over the samples one
+ * ray evaluates (up to its termination).  An FP32 evaluation can take the
+ * other side of a ReLU kink when this margin is below its rounding error, and
+ * then its gradient differs from the FP64 one by that unit's whole term. */
+double orc_ray_relu_margin(const float *table, const float *weights,
+                           const uint32_t *occ, const float o[3], const float d[3]) {
+  params_t p = {table, weights, -1, -1, 0.0};
+  sample_pos s[ORC_MAX_STEPS];
+  int n = ray_samples(occ, o, d, s);
+  double dd[3] = {d[0], d[1], d[2]}, sh[16];
+  orc_sh4(dd, sh);
+  double T = 1.0, margin = INFINITY;
+  for (int i = 0; i < n; ++i) {
+    double x[3] = {s[i].x[0], s[i].x[1], s[i].x[2]}, feat[32];
+    mlp_trace tr;
+    encode_p(&p, x, feat);
+    mlp_p(&p, feat, sh, &tr);
+    for (int u = 0; u < 64; ++u) {
+      margin = fmin(margin, fabs(tr.h1p[u]));
+      margin = fmin(margin, fabs(tr.h3p[u]));
+      margin = fmin(margin, fabs(tr.h4p[u]));
+    }
+    T *= exp(-tr.sigma * (double)ORC_DELTA);
+    if (T < ORC_T_STOP) break;
+  }
+  return margin;
+}
+
 void orc_render_pixels(const float *table, const float *weights,
                        const uint32_t *occ, const orc_camera *cam,
                        const uint32_t *pix, int n, double *rgb, double *depth,

@@ -153,6 +153,12 @@ int orc_frameprep(const uint8_t *rgb_a, const float *depth_a,
                   uint8_t *y, int y_stride, uint8_t *u, int u_stride,
                   uint8_t *v, int v_stride, int mode);
 
+/* Smallest |pre-activation| of the ReLU layers over one ray's evaluated
+ * samples (a kink margin: FP32 and FP64 gradients may differ by a whole
+ * unit's term when it is below FP32 rounding). */
+double orc_ray_relu_margin(const float *table, const float *weights,
+                           const uint32_t *occ, const float o[3], const float d[3]);
+
 /* Text overlay: RenderTextContext::render_string_to_frame
  * (src/base/video/render_text.cc:35-111) on a tightly packed RGB24 frame,
  * with each character's FreeType glyph given as a table entry (bitmap_left,

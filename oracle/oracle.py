@@ -52,6 +52,7 @@ def _declare(L):
                                          _p(C.c_uint32), _p(C.c_int32), _p(C.c_int16))
     sig = {
         "orc_mix64": (C.c_uint64, [C.c_uint64]),
+        "orc_ray_relu_margin": (C.c_double, [f32p, f32p, u32p, f32p, f32p]),
         "orc_render_string": (None, [u8p, C.c_uint32, C.c_uint32, _p(Glyph), u8p, C.c_int,
                                      C.c_char_p]),
         "orc_rand_u64": (C.c_uint64, [C.c_uint64, C.c_uint64, C.c_uint64]),
@@ -158,6 +159,14 @@ def render_ray(table, weights, occ, o, d):
     lib().orc_render_ray(ptr(table, C.c_float), ptr(weights, C.c_float), ptr(occ, C.c_uint32),
                          ptr(o, C.c_float), ptr(d, C.c_float), C.byref(out))
     return np.array(out.rgb), out.depth, out.transmittance, out.n_samples
+
+
+def ray_relu_margin(table, weights, occ, o, d):
+    """Smallest |pre-activation| of the ReLU layers over one ray's samples."""
+    o = np.ascontiguousarray(o, np.float32)
+    d = np.ascontiguousarray(d, np.float32)
+    return lib().orc_ray_relu_margin(ptr(table, C.c_float), ptr(weights, C.c_float),
+                                     ptr(occ, C.c_uint32), ptr(o, C.c_float), ptr(d, C.c_float))
 
 
 def render_pixels(table, weights, occ, cam, pix, threads=8):

@@ -1,18 +1,35 @@
 """Differentiable render on the GPU: forward and gradients against the FP64
 oracle (itself pinned by finite differences), the autograd binding, Adam's
-closed-form first step and a loss that falls on the config-3 setup."""
+closed-form first step and a loss that falls on the config-3 setup.
+
+Gradient tolerances: a ReLU whose pre-activation lies closer to zero than the
+FP32 forward's rounding error can switch sides, and the FP32 gradient then
+differs from the FP64 one by that unit's whole term for that sample (measured:
+the two outlier rays of 60 had margins 2e-7 and 1.2e-6).  Element-wise
+comparisons therefore use rays whose smallest |pre-activation| over every
+evaluated sample (the oracle's kink margin) exceeds 1e-5, a thousand times the
+FP32 error of the pre-activations; whole-batch comparisons use relative L2
+norms."""
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
 
 
+KINK_MARGIN = 1e-5
+
+
 @pytest.fixture(scope="module")
 def rays(oracle, scene1):
+    """3000 training rays; keep = those with samples and a kink margin above
+    KINK_MARGIN."""
     t, w, o = scene1
     ro, rd, _, _ = oracle.training_rays(0, 0, 3000)
     ns = np.array([oracle.render_ray(t, w, o, ro[i], rd[i])[3] for i in range(len(ro))])
-    keep = np.flatnonzero(ns > 0)[:300]
+    hit = np.flatnonzero(ns > 0)
+    margin = np.array([oracle.ray_relu_margin(t, w, o, ro[i], rd[i]) for i in hit])
+    keep = hit[margin > KINK_MARGIN]
+    assert len(keep) >= 100
     return ro, rd, keep
 
 
@@ -40,16 +57,22 @@ def test_backward_matches_oracle_gradient(oracle, scene1, trainer, rays):
     rgb = trainer.forward(ro, rd)
     d_rgb = 2.0 * (rgb - target) / (3 * len(ro))
     gt, gw = trainer.backward(d_rgb)
-    # MLP weights: every layer within 1e-3 of the largest oracle gradient
-    for a, b in [(0, 2048), (2048, 3072), (3072, 5056), (5056, 9152), (9152, 9344)]:
+    layers = [(0, 2048), (2048, 3072), (3072, 5056), (5056, 9152), (9152, 9344)]
+    for a, b in layers + [(None, None)]:
+        x, r = (gw[a:b], gw_ref[a:b]) if a is not None else (gt, gt_ref)
+        rel = np.linalg.norm(x - r) / np.linalg.norm(r)
+        mx = np.abs(x - r).max() / np.abs(r).max()
+        print(f"grad {a}:{b} rel-l2 {rel:.2e} max/scale {mx:.2e}", flush=True)
+    # MLP weights: every layer within 1e-4 of its largest oracle gradient
+    for a, b in layers:
         scale = np.abs(gw_ref[a:b]).max()
-        assert np.abs(gw[a:b] - gw_ref[a:b]).max() <= 1e-3 * scale, (a, b)
-    # table: same support, values within 1e-3 of the largest gradient
-    assert np.array_equal(gt != 0, gt_ref != 0) or np.abs(gt[gt_ref == 0]).max() == 0
+        assert np.abs(gw[a:b] - gw_ref[a:b]).max() <= 1e-4 * scale, (a, b)
+    # table: outside the oracle's support only FP32 cancellation residue, and
+    # values within 1e-4 of the largest gradient
     scale = np.abs(gt_ref).max()
-    assert np.abs(gt - gt_ref).max() <= 1e-3 * scale
-    cos = float(gt @ gt_ref / np.linalg.norm(gt) / np.linalg.norm(gt_ref))
-    assert cos > 0.99999
+    assert np.abs(gt[gt_ref == 0]).max() <= 1e-6 * scale
+    assert np.abs(gt - gt_ref).max() <= 1e-4 * scale
+    assert np.linalg.norm(gt - gt_ref) <= 1e-4 * np.linalg.norm(gt_ref)
 
 
 def test_autograd_binding(nes, oracle, scene1, trainer, rays):
@@ -101,14 +124,25 @@ def test_batch_loss_matches_oracle_and_adam_first_step(nes, oracle, scene1):
         tgt = targets.reshape(16, -1, 3)[pose, pix]
         loss_ref, _, gw_ref = oracle.loss_grad(ts, ws, o, ro, rd, tgt)
         assert abs(loss - loss_ref) <= 1e-4 * loss_ref
-        assert np.abs(gw - gw_ref).max() <= 1e-3 * np.abs(gw_ref).max()
+        # whole batch, kinks included: relative L2 per layer.  The student's
+        # layer-1 gradient is a sum of ~1e5 terms of random sign, so one
+        # flipped kink moves it by ~1/sqrt(1e5) = 3e-3 of its norm.
+        for a, b in [(0, 2048), (2048, 3072), (3072, 5056), (5056, 9152), (9152, 9344)]:
+            rel = np.linalg.norm(gw[a:b] - gw_ref[a:b]) / np.linalg.norm(gw_ref[a:b])
+            print(f"batch grad {a}:{b} rel-l2 {rel:.2e}", flush=True)
+            assert rel <= 2e-2, (a, b, rel)
         l0, st = tr.step()
         assert abs(l0 - loss) <= 1e-5 * loss and st.step == 1
         t1, w1 = tr.get_params()
-    # first Adam step from zero moments: p -= lr * g / (|g| + eps)
+    # first Adam step from zero moments: p -= lr * g / (|g| + eps).  The step
+    # re-evaluates the gradient (atomics in another order), and where |g| is
+    # near eps the update is steep in g, so those entries are only held to a
+    # step no longer than lr.
     for p0, p1, g in ((ts, t1, gt), (ws, w1, gw)):
         want = p0 - 1e-2 * g / (np.abs(g) + 1e-15)
-        assert np.abs(p1 - want).max() <= 1e-6
+        firm = np.abs(g) > 1e-12
+        assert np.abs(p1 - want)[firm].max() <= 1e-6
+        assert np.abs(p1 - p0).max() <= 1e-2 * (1 + 1e-5)
 
 
 def test_config3_training_reduces_loss(nes, oracle, scene1):

@@ -50,3 +50,51 @@ def test_training_rays_are_the_sampler(oracle):
     assert np.allclose(np.linalg.norm(ro - 0.5, axis=1), 2.0, atol=1e-5)
     assert np.allclose(np.linalg.norm(rd, axis=1), 1.0, atol=1e-6)
     assert pose.max() < 16 and pix.max() < 640 * 480
+
+
+def test_relu_margin(oracle, scene1):
+    """The kink margin is +inf for a ray that misses the cube and never above
+    the smallest |layer-1 pre-activation| of the ray's first sample."""
+    t, w, o = scene1
+    miss = oracle.ray_relu_margin(t, w, o, np.array([5.0, 5.0, 5.0], np.float32),
+                                  np.array([1.0, 0.0, 0.0], np.float32))
+    assert miss == np.inf
+    ro, rd, _, _ = oracle.training_rays(0, 0, 400)
+    checked = 0
+    for i in range(len(ro)):
+        if oracle.render_ray(t, w, o, ro[i], rd[i])[3] == 0:
+            continue
+        m = oracle.ray_relu_margin(t, w, o, ro[i], rd[i])
+        # first occupied sample position, by marching like the oracle
+        x = first_sample(oracle, o, ro[i], rd[i])
+        d1 = w[:2048].astype(np.float64).reshape(64, 32) @ oracle.encode(t, np.clip(x, 0, 1))
+        assert 0 < m <= np.abs(d1).min()
+        checked += 1
+        if checked == 5:
+            break
+    assert checked == 5
+
+
+def first_sample(oracle, occ, o, d):
+    import ctypes as C
+    L = oracle.lib()
+    f32 = C.POINTER(C.c_float)
+    o = np.ascontiguousarray(o, np.float32)
+    d = np.ascontiguousarray(d, np.float32)
+    ts, te = C.c_float(), C.c_float()
+    n = L.orc_ray_box(o.ctypes.data_as(f32), d.ctypes.data_as(f32), C.byref(ts), C.byref(te))
+    for k in range(n):
+        t = L.orc_step_t(k, ts.value)
+        x = np.zeros(3, np.float32)
+        step_pos(L, o, d, t, x)
+        if L.orc_occupied(occ.ctypes.data_as(C.POINTER(C.c_uint32)), x.ctypes.data_as(f32)):
+            return x.astype(np.float64)
+    raise AssertionError("ray has no sample")
+
+
+def step_pos(L, o, d, t, x):
+    import ctypes as C
+    f32 = C.POINTER(C.c_float)
+    L.orc_step_pos.restype = None
+    L.orc_step_pos.argtypes = [f32, f32, C.c_float, f32]
+    L.orc_step_pos(o.ctypes.data_as(f32), d.ctypes.data_as(f32), t, x.ctypes.data_as(f32))
