@@ -183,37 +183,6 @@ __device__ __forceinline__ PartLevels part_levels(int part) {
   return p;
 }
 
-// Corners (x, y, z) and (x+1, y, z) share an aligned table word pair when
-// the second index is the first with bit 0 flipped (hashed levels: x even;
-// dense levels: first index even); such pairs are fetched with one load.
-template <class Tab> struct PairFeat;
-template <> struct PairFeat<_Float16> {
-  static __device__ __forceinline__ void load(const _Float16 *__restrict__ t, uint32_t e0,
-                                              uint32_t e1, uint32_t &v0, uint32_t &v1) {
-    const uint32_t *w = reinterpret_cast<const uint32_t *>(t);
-    const uint2 pr = __ldg(reinterpret_cast<const uint2 *>(w + (e0 & ~1u)));
-    const bool odd = e0 & 1u;
-    v0 = odd ? pr.y : pr.x;
-    if (e1 == (e0 ^ 1u))
-      v1 = odd ? pr.x : pr.y;
-    else
-      v1 = __ldg(w + e1);
-  }
-};
-template <> struct PairFeat<float> {
-  static __device__ __forceinline__ void load(const float *__restrict__ t, uint32_t e0,
-                                              uint32_t e1, float2 &v0, float2 &v1) {
-    const float2 *w = reinterpret_cast<const float2 *>(t);
-    const float4 pr = __ldg(reinterpret_cast<const float4 *>(w + (e0 & ~1u)));
-    const bool odd = e0 & 1u;
-    v0 = odd ? make_float2(pr.z, pr.w) : make_float2(pr.x, pr.y);
-    if (e1 == (e0 ^ 1u))
-      v1 = odd ? make_float2(pr.x, pr.y) : make_float2(pr.z, pr.w);
-    else
-      v1 = __ldg(w + e1);
-  }
-};
-
 template <class Tab>
 __device__ __forceinline__ float2 encode_one(const Tab *__restrict__ table, const PartLevels &lp,
                                              int q, const float xc[3]) {
@@ -237,9 +206,13 @@ __device__ __forceinline__ float2 encode_one(const Tab *__restrict__ table, cons
       const uint32_t hash = (x ^ (y * kPrime1) ^ (z * kPrime2)) & ((1u << kLog2T) - 1u);
       e[h] = lp.off[q] + (lp.dense[q] ? dense : hash);
     }
-    // pairing is decided on the global entry index, so it holds for any
-    // level offset (odd offsets just pair less often)
-    PairFeat<Tab>::load(table, e[0], e[1], v[c], v[c + 1]);
+    // one 4-byte (FP16) or 8-byte (FP32) gather per corner: fetching the
+    // (x, x+1) pair as one wider load when it shares an aligned word costs a
+    // second instruction anyway for the lanes that do not pair, and the
+    // wider lanes were measured slower (divergent gathers are bound by the
+    // vector L1's per-lane address rate, not by the request count)
+    v[c] = RawFeat<Tab>::load(table, e[0]);
+    v[c + 1] = RawFeat<Tab>::load(table, e[1]);
   }
   const float gx = 1.0f - fr[0], gy = 1.0f - fr[1], gz = 1.0f - fr[2];
   float a0 = 0.0f, a1 = 0.0f;
