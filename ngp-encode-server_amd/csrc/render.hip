@@ -551,7 +551,7 @@ int render_launch_pass(const SceneDev &scene, RenderWork &w, int precision, int 
   NES_HIP_TRY(hipGetLastError());
   if (ev_eval_begin) NES_HIP_TRY(hipEventRecord(ev_eval_begin, stream));
   // fixed grids with grid-stride loops: the sample count stays on the device
-  const uint32_t bpp = 512;
+  const uint32_t bpp = part_blocks(cap);
   if (precision == NES_PRECISION_FP32) {
     encode_parts<float><<<bpp * kParts, 256, 0, stream>>>(w.sample_pos.as<float4>(), cursor, cap,
                                                            scene.table_f32, w.feat.as<float4>(),
@@ -599,7 +599,7 @@ int launch_march_emit(const uint32_t *occ, RenderWork &w, int k0, int k1, hipStr
 }
 
 int launch_encode_f32(const float *table, RenderWork &w, hipStream_t stream) {
-  const uint32_t bpp = 512;
+  const uint32_t bpp = part_blocks(w.capacity);
   encode_parts<float><<<bpp * kParts, 256, 0, stream>>>(w.sample_pos.as<float4>(),
                                                          w.total.as<uint32_t>(), w.capacity,
                                                          table, w.feat.as<float4>(), bpp);

@@ -560,7 +560,7 @@ static int trainer_backward(nes_trainer *t, uint32_t n_rays) {
   ma.grad_w = t->grads + t->n_table;
   mlp_train<true><<<256, 256, 0, s>>>(ma);
   NES_HIP_TRY(hipGetLastError());
-  const uint32_t bpp = 512;
+  const uint32_t bpp = part_blocks(w.capacity);
   scatter_parts<<<bpp * kParts, 256, 0, s>>>(w.sample_pos.as<float4>(), t->dfeat.as<float4>(),
                                              w.total.as<uint32_t>(), w.capacity, t->grads, bpp);
   NES_HIP_TRY(hipGetLastError());
