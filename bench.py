@@ -79,6 +79,34 @@ def cpu_baseline(N, table, weights, occ, ctx, views, budget_s):
     return base, psnr, total_rays
 
 
+def cpu_frameprep_baseline(threads, w=1280, h=720, runs=100, warmup=10):
+    """The repo's CPU frame prep (composite + RGB24 -> YUV420P restatement,
+    filter built per frame as SwsContextManager does) on one 1280x720 pair:
+    median ms per frame of `runs` after `warmup`, on 1 thread (one
+    process_frame_thread per eye) and on `threads` threads (rows split)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    rng = np.random.default_rng(0)
+    a = rng.integers(0, 256, (h, w, 3), dtype=np.uint8)
+    b = rng.integers(0, 256, (h, w, 3), dtype=np.uint8)
+    da = rng.uniform(0.1, 10.0, (h, w)).astype(np.float32)
+    db = rng.uniform(0.1, 10.0, (h, w)).astype(np.float32)
+    out = {}
+    for key, n in (("ms_per_frame_1t", 1), ("ms_per_frame_mt", threads)):
+        O.set_frameprep_threads(n)
+        times = []
+        for i in range(warmup + runs):
+            t0 = time.perf_counter()
+            O.frameprep(a, da, b, db)
+            if i >= warmup:
+                times.append((time.perf_counter() - t0) * 1e3)
+        out[key] = float(np.median(times))
+    O.set_frameprep_threads(1)
+    out.update({"threads": threads, "kind": "port", "runs": runs, "warmup": warmup,
+                "sample": f"2 x {w}x{h} RGB24 + f32 depth -> composite -> YUV420P"})
+    return out
+
+
 def serving_leg(N, ctx, seconds):
     """8 Poisson clients at 72 Hz, stereo, per-eye {640x480, 1440x1600, 2064x2208}."""
     with N.Server(ctx) as srv:
@@ -208,6 +236,12 @@ def main():
             result["cpu_baseline"] = base
             result["psnr_db"] = round(psnr, 2)
             result["psnr_rays"] = nr
+            if "frameprep" in result:
+                cf = cpu_frameprep_baseline(base["cores"])
+                result["frameprep"]["cpu"] = cf
+                result["frameprep"]["gpu_vs_cpu_1t"] = round(
+                    cf["ms_per_frame_1t"] / (result["frameprep"]["ms_per_launch"] /
+                                             result["frameprep"]["frames_per_launch"]), 1)
     print(json.dumps(result))
 
 

@@ -902,6 +902,12 @@ static uint8_t vchroma(const int16_t *coef, int n, const int *src, int x86) {
 
 #define ORC_MAX_VTAPS 16
 
+/* Threads for the frame-prep row loops (rows are independent, so the output
+ * does not depend on it); 1 = one thread per frame, as process_frame_thread
+ * runs one per eye (src/main.cpp:274-282). */
+static int g_fp_threads = 1;
+void orc_set_frameprep_threads(int n) { g_fp_threads = n > 0 ? n : 1; }
+
 static int yuv420p_from_planes15(int w, int h, const int16_t *u15,
                                  const int16_t *v15, uint8_t *u, int u_stride,
                                  uint8_t *v, int v_stride, int mode) {
@@ -914,8 +920,11 @@ static int yuv420p_from_planes15(int w, int h, const int16_t *u15,
     free(pos);
     return -1;
   }
-  int su[ORC_MAX_VTAPS], sv[ORC_MAX_VTAPS];
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(g_fp_threads)
+#endif
   for (int k = 0; k < ch; ++k) {
+    int su[ORC_MAX_VTAPS], sv[ORC_MAX_VTAPS];
     /* swscale.c: output rows dstY >= dstH-2 use the C output functions */
     int x86 = (mode == ORC_SWS_MODE_X86) && (2 * k < h - 2);
     for (int i = 0; i < cw; ++i) {
@@ -941,6 +950,9 @@ int orc_rgb24_to_yuv420p(const uint8_t *rgb, int rgb_stride, int w, int h,
   const int cw = w >> 1;
   int16_t *u15 = (int16_t *)malloc(sizeof(int16_t) * (size_t)cw * h);
   int16_t *v15 = (int16_t *)malloc(sizeof(int16_t) * (size_t)cw * h);
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(g_fp_threads)
+#endif
   for (int r = 0; r < h; ++r) {
     const uint8_t *p = rgb + (size_t)r * rgb_stride;
     for (int x = 0; x < w; ++x) {
@@ -985,6 +997,9 @@ int orc_gray8_to_yuv420p(const uint8_t *g, int g_stride, int w, int h,
 void orc_composite(const uint8_t *rgb_a, const float *depth_a,
                    const uint8_t *rgb_b, const float *depth_b, int w, int h,
                    int rgb_stride, uint8_t *rgb_out, float *depth_out) {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(g_fp_threads)
+#endif
   for (int r = 0; r < h; ++r)
     for (int x = 0; x < w; ++x) {
       size_t pi = (size_t)r * w + x;

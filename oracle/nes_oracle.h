@@ -148,6 +148,9 @@ int orc_gray8_to_yuv420p(const uint8_t *g, int g_stride, int w, int h,
 void orc_composite(const uint8_t *rgb_a, const float *depth_a,
                    const uint8_t *rgb_b, const float *depth_b, int w, int h,
                    int rgb_stride, uint8_t *rgb_out, float *depth_out);
+/* Threads used by the frame-prep row loops (default 1); outputs do not
+ * depend on it. */
+void orc_set_frameprep_threads(int n);
 int orc_frameprep(const uint8_t *rgb_a, const float *depth_a,
                   const uint8_t *rgb_b, const float *depth_b, int w, int h,
                   uint8_t *y, int y_stride, uint8_t *u, int u_stride,

@@ -52,6 +52,7 @@ def _declare(L):
                                          _p(C.c_uint32), _p(C.c_int32), _p(C.c_int16))
     sig = {
         "orc_mix64": (C.c_uint64, [C.c_uint64]),
+        "orc_set_frameprep_threads": (None, [C.c_int]),
         "orc_ray_relu_margin": (C.c_double, [f32p, f32p, u32p, f32p, f32p]),
         "orc_render_string": (None, [u8p, C.c_uint32, C.c_uint32, _p(Glyph), u8p, C.c_int,
                                      C.c_char_p]),
@@ -273,6 +274,10 @@ def render_string(rgb, glyphs, bitmaps, position, text):
                             bm.ctypes.data_as(C.POINTER(C.c_uint8)), position,
                             text.encode("latin-1"))
     return rgb
+
+
+def set_frameprep_threads(n):
+    lib().orc_set_frameprep_threads(int(n))
 
 
 def frameprep(rgb_a, depth_a, rgb_b=None, depth_b=None, mode=MODE_X86):

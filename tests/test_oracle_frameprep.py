@@ -155,3 +155,22 @@ def test_odd_sizes_rejected(oracle):
     img = np.zeros((5, 4, 3), np.uint8)
     with pytest.raises(ValueError):
         oracle.rgb24_to_yuv420p(img)
+
+
+def test_threaded_rows_match_single_thread(oracle):
+    """Row-parallel frame prep (the all-cores CPU baseline) gives the same
+    planes as one thread."""
+    rng = np.random.default_rng(11)
+    h, w = 98, 130
+    a = rng.integers(0, 256, (h, w, 3), dtype=np.uint8)
+    b = rng.integers(0, 256, (h, w, 3), dtype=np.uint8)
+    da = rng.uniform(0.1, 10, (h, w)).astype(np.float32)
+    db = rng.uniform(0.1, 10, (h, w)).astype(np.float32)
+    for mode in (0, 1):
+        oracle.set_frameprep_threads(1)
+        one = oracle.frameprep(a, da, b, db, mode)
+        oracle.set_frameprep_threads(4)
+        four = oracle.frameprep(a, da, b, db, mode)
+        oracle.set_frameprep_threads(1)
+        for p, q in zip(one, four):
+            assert np.array_equal(p, q)
