@@ -245,7 +245,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
 // FP16 MFMA MLPs: lane (g, j) of a 16-sample tile loads parts 2g and 2g+1 of
 // sample j -- the layer-1 B operand with K-slot s of group g holding feature
 // part_feature(2g + s/4, s%4) (baked into the W1 fragments).
-__global__ __launch_bounds__(256) void mlp_mixed(const float4 *__restrict__ pos,
+__global__ __launch_bounds__(256) void mlp_mixed(const uint32_t *__restrict__ samples,
                                                  const uint2 *__restrict__ feat,
                                                  const uint32_t *__restrict__ n_ptr,
                                                  uint32_t stride,
@@ -267,7 +267,7 @@ __global__ __launch_bounds__(256) void mlp_mixed(const float4 *__restrict__ pos,
     const uint32_t sc = min(s, n - 1);  // n >= 1 inside the loop
     const uint2 fa = feat[(size_t)(2 * g) * stride + sc];
     const uint2 fb = feat[(size_t)(2 * g + 1) * stride + sc];
-    const RayData r = load_ray(rays, __float_as_uint(pos[sc].w));
+    const RayData r = load_ray(rays, samples[sc] >> 7);  // record = slot << 7 | step
     const uint32_t words[4] = {fa.x, fa.y, fb.x, fb.y};
     half8 b1;
 #pragma unroll
@@ -276,12 +276,12 @@ __global__ __launch_bounds__(256) void mlp_mixed(const float4 *__restrict__ pos,
       b1[2 * k] = h.x;
       b1[2 * k + 1] = h.y;
     }
-    const half4 shv = sh4_quarter(r.d, g);
     float4v d1[4];
 #pragma unroll
     for (int m = 0; m < 4; ++m) d1[m] = mfma(wf[64 * m], b1, zero);
     float4v d2 = mfma(wf[64 * 4], pack_relu(d1[0], d1[1]), zero);
     d2 = mfma(wf[64 * 5], pack_relu(d1[2], d1[3]), d2);
+    const half4 shv = sh4_quarter(r.d, g);
     half8 b3;
     b3[0] = (_Float16)d2[0]; b3[1] = (_Float16)d2[1];
     b3[2] = (_Float16)d2[2]; b3[3] = (_Float16)d2[3];
@@ -567,7 +567,7 @@ int render_launch_pass(const SceneDev &scene, RenderWork &w, int precision, int 
                                                               w.feat.as<uint2>(), bpp);
     NES_HIP_TRY(hipGetLastError());
     if (ev_encode_end) NES_HIP_TRY(hipEventRecord(ev_encode_end, stream));
-    mlp_mixed<<<256 * 8, 256, 0, stream>>>(w.sample_pos.as<float4>(), w.feat.as<uint2>(), cursor,
+    mlp_mixed<<<256 * 8, 256, 0, stream>>>(w.samples.as<uint32_t>(), w.feat.as<uint2>(), cursor,
                                            cap, w.rays.as<RayData>(),
                                            reinterpret_cast<const half8 *>(scene.wfrag),
                                            w.sample_out.as<float4>());
