@@ -80,13 +80,17 @@ int nes_orbit_camera(double azimuth_deg, double elevation_deg, double radius,
 
 typedef struct nes_ctx nes_ctx;
 
-/* Uploads the scene to the device.  hbm_budget_bytes bounds the context's
- * work buffers (0 = 16 GiB). Replaces connecting to a renderer
- * (socket_manage_thread, src/server.cpp:216-283). */
+/* Uploads the scene to the device.  hbm_budget_bytes bounds the device
+ * buffers of the context's own renders (nes_render*, 0 = 16 GiB): a render
+ * whose ray and sample buffers would exceed it fails with NES_ENOMEM and
+ * releases them.  Replaces connecting to a renderer (socket_manage_thread,
+ * src/server.cpp:216-283).  Destroy servers and wire endpoints built on a
+ * context before the context. */
 int nes_ctx_create(const nes_scene_desc *scene, uint64_t hbm_budget_bytes,
                    nes_ctx **out);
 int nes_ctx_destroy(nes_ctx *ctx);
-/* Replace the scene parameters (e.g. after training). */
+/* Replace the scene parameters (e.g. after training) in place.  Batches a
+ * server on this context has in flight may see the old or the new values. */
 int nes_ctx_update_scene(nes_ctx *ctx, const nes_scene_desc *scene);
 
 /* --------------------------------------------------------------- render */

@@ -42,7 +42,11 @@ struct RenderWork {
   int capacity_precision = -1;    // feature layout the buffers were sized for
   size_t out_px = 0;
   std::vector<ViewDesc> host_views;
+  uint64_t budget = 0;            // device bytes the buffers may hold, 0 = no limit
 };
+
+// Device bytes currently held by a RenderWork's buffers.
+size_t work_device_bytes(const RenderWork &w);
 
 struct SceneDev {
   float *table_f32 = nullptr;      // entries x 2

@@ -311,50 +311,71 @@ struct OutPtrs {
   uint16_t *nsamp;
 };
 
+// One lane per ray slot.  A wavefront's rays (one 8x8 tile) own one
+// contiguous range of the sample list (march_emit reserved it with a single
+// atomic), so the wavefront stages that range through LDS in coalesced
+// chunks and each lane then composites its own segment, in order, from LDS.
+constexpr int kCompChunk = 256;
+
 __global__ __launch_bounds__(256) void composite(
     uint32_t n_slots, const RayData *__restrict__ rays, RayAcc *acc,
     const uint32_t *__restrict__ counts, const uint32_t *__restrict__ offsets,
     const uint32_t *__restrict__ samples, const float4 *__restrict__ sout,
     const uint32_t *__restrict__ n_ptr, uint32_t capacity, int keep, int last, OutPtrs o) {
-  const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
-  if (slot >= n_slots) return;
+  __shared__ float4 s_out[4][kCompChunk];
+  __shared__ uint32_t s_k[4][kCompChunk];
   if (*n_ptr > capacity) return;  // overflowed pass: the host re-renders
-  RayAcc a = acc[slot];
-  const uint32_t cnt = counts[slot];
-  if (!a.done && cnt) {
-    const uint32_t off = offsets[slot];
-    const float ts = load_ray(rays, slot).t_start;
-    // samples are fetched eight at a time (independent loads), composited in order
-    for (uint32_t j0 = 0; j0 < cnt && !a.done; j0 += 8) {
-      float4 sb[8];
-      uint32_t kb[8];
+  const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = lane_id(), wave = (int)(threadIdx.x >> 6);
+  const bool valid = slot < n_slots;
+  RayAcc a{};
+  uint32_t cnt = 0, off = 0;
+  if (valid) {
+    a = acc[slot];
+    cnt = counts[slot];
+    off = offsets[slot];
+  }
+  const bool live = valid && !a.done && cnt;
+  const float ts = live ? load_ray(rays, slot).t_start : 0.0f;
+  uint32_t lo = live ? off : 0xffffffffu, hi = live ? off + cnt : 0u;
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const uint32_t j = min(j0 + (uint32_t)u, cnt - 1);
-        sb[u] = sout[off + j];
-        kb[u] = samples[off + j];
-      }
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        if (j0 + (uint32_t)u >= cnt || a.done) break;
-        const float4 s = sb[u];
-        const float t = step_t((int)(kb[u] & 127u), ts);
-        const float sd = s.x * kDelta;
+  for (int m = 32; m > 0; m >>= 1) {
+    lo = min(lo, (uint32_t)__shfl_xor((int)lo, m));
+    hi = max(hi, (uint32_t)__shfl_xor((int)hi, m));
+  }
+  for (uint32_t c0 = lo; c0 < hi; c0 += kCompChunk) {  // wavefront-uniform
+    const uint32_t cn = min((uint32_t)kCompChunk, hi - c0);
+    for (uint32_t i = (uint32_t)lane; i < cn; i += 64) {
+      s_out[wave][i] = sout[c0 + i];
+      s_k[wave][i] = samples[c0 + i];
+    }
+    wave_lds_fence();
+    if (live && !a.done) {
+      const uint32_t b = max(off, c0), e = min(off + cnt, c0 + cn);
+      for (uint32_t jj = b; jj < e; ++jj) {
+        const float4 sm = s_out[wave][jj - c0];
+        const float t = step_t((int)(s_k[wave][jj - c0] & 127u), ts);
+        const float sd = sm.x * kDelta;
         const float alpha = -expm1f(-sd);
         const float w = a.T * alpha;
-        a.r = fmaf(w, s.y, a.r);
-        a.g = fmaf(w, s.z, a.g);
-        a.b = fmaf(w, s.w, a.b);
+        a.r = fmaf(w, sm.y, a.r);
+        a.g = fmaf(w, sm.z, a.g);
+        a.b = fmaf(w, sm.w, a.b);
         a.depth = fmaf(w, t, a.depth);
         a.T *= expf(-sd);
         a.n_samples++;
-        if (a.T < kTStop) a.done = 1;
+        if (a.T < kTStop) {
+          a.done = 1;
+          break;
+        }
       }
     }
-    // the state carries into the next pass; after the last pass only a
-    // caller that reads it again (keep: the training backward) stores it
-    if (!last || keep) acc[slot] = a;
+    wave_lds_fence();  // every lane is done reading before the next chunk lands
   }
+  if (!valid) return;
+  // the state carries into the next pass; after the last pass only a caller
+  // that reads it again (keep: the training backward) stores it
+  if (live && (!last || keep)) acc[slot] = a;
   if (last && a.pad != 0xffffffffu) {
     const uint32_t p = a.pad;
     const float c[3] = {a.r, a.g, a.b};
@@ -627,6 +648,30 @@ int render_prepare_capacity(RenderWork &w, int precision) {
   return render_reserve_samples(w, precision, want);
 }
 
+size_t work_device_bytes(const RenderWork &w) {
+  size_t b = 0;
+  for (const DevBuf *d : {&w.views, &w.rays, &w.acc, &w.counts, &w.offsets, &w.samples,
+                          &w.sample_out, &w.sample_pos, &w.feat, &w.out, &w.total})
+    b += d->bytes;
+  return b;
+}
+
+// Over budget: drop every buffer (they only grow) so that a later, smaller
+// render starts from nothing, and report NES_ENOMEM.
+static int check_budget(RenderWork &w) {
+  if (w.budget && work_device_bytes(w) > w.budget) {
+    for (DevBuf *d : {&w.views, &w.rays, &w.acc, &w.counts, &w.offsets, &w.samples,
+                      &w.sample_out, &w.sample_pos, &w.feat, &w.out, &w.total})
+      d->release();
+    w.capacity = 0;
+    w.capacity_precision = -1;
+    w.n_views = 0;
+    set_last_error("render buffers exceed the context's HBM budget");
+    return NES_ENOMEM;
+  }
+  return NES_OK;
+}
+
 int render_enqueue(const SceneDev &scene, RenderWork &w, int precision,
                    const nes_camera *views, int n_views, int steps_per_pass,
                    hipStream_t stream, nes_render_stats *stats) {
@@ -635,6 +680,7 @@ int render_enqueue(const SceneDev &scene, RenderWork &w, int precision,
   const int spp = steps_per_pass == 0 ? kMaxSteps : std::min(steps_per_pass, kMaxSteps);
   NES_TRY(render_setup(w, views, n_views, stream));
   NES_TRY(render_prepare_capacity(w, precision));
+  NES_TRY(check_budget(w));
   double ms_march = 0, ms_eval = 0, ms_comp = 0, ms_enc = 0;
   uint64_t n_samples = 0;
   NES_HIP_TRY(hipEventRecord(w.ev[0], stream));
@@ -657,6 +703,7 @@ int render_enqueue(const SceneDev &scene, RenderWork &w, int precision,
         return NES_ENOMEM;
       }
       NES_TRY(render_reserve_samples(w, precision, (size_t)n + n / 8 + 1024));
+      NES_TRY(check_budget(w));
     }
     float a = 0, b = 0, c = 0, e = 0;
     hipEventElapsedTime(&a, w.ev[1], w.ev[2]);
@@ -712,6 +759,7 @@ int nes_ctx_create(const nes_scene_desc *scene, uint64_t hbm_budget_bytes, nes_c
   NES_TRY(ensure_device());
   nes_ctx *c = new nes_ctx();
   c->budget = hbm_budget_bytes ? hbm_budget_bytes : (16ull << 30);
+  c->work.budget = c->budget;
   int rc = scene_upload(c->scene, scene);
   if (rc == NES_OK && hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) rc = NES_EIO;
   if (rc != NES_OK) {

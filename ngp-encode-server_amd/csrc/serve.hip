@@ -143,11 +143,20 @@ struct nes_server {
 
 namespace nes {
 
-static size_t slot_bytes(uint32_t max_rays, uint32_t samples_per_ray) {
-  // ray state + counts/offsets + per-pixel outputs + YUV + samples
+// Device bytes one batch slot holds for max_rays ray slots, as the render
+// allocates them: per ray slot the ray state (RayData, RayAcc), counts and
+// offsets, per-pixel outputs (RGB24, depth, float RGB, sample count) and the
+// YUV420P frame; per sample (the initial buffer holds 4 per ray slot, at
+// least 2^20) the record, position, MLP output and the 8 feature planes;
+// plus the 1/8 growth slack of every device buffer.  A batch that marches
+// more samples grows its buffers past this (up to 128 per ray).
+static size_t slot_bytes(uint32_t max_rays, int precision) {
   const size_t r = max_rays;
-  return r * (sizeof(RayData) + sizeof(RayAcc) + 8 + 21 + 2) +
-         r * (size_t)samples_per_ray * (4 + 16) + (1u << 20);
+  const size_t cap = std::max<size_t>(r * 4, (size_t)1 << 20);
+  const size_t feat = precision == NES_PRECISION_FP32 ? sizeof(float4) : sizeof(uint2);
+  const size_t per_ray = sizeof(RayData) + sizeof(RayAcc) + 8 + (3 + 4 + 12 + 2) + 2;
+  const size_t per_sample = 4 + sizeof(float4) + sizeof(float4) + feat * 8;  // 8 XCD part planes
+  return (r * per_ray + cap * per_sample) / 8 * 9 + ((size_t)1 << 20);
 }
 
 // Launch frame prep of every view of a finished render and the D2H copies.
@@ -329,7 +338,7 @@ int nes_server_create(nes_ctx *ctx, const nes_server_config *cfg, nes_server **o
   if (s->cfg.sws_mode != NES_SWS_X86 && s->cfg.sws_mode != NES_SWS_C) return NES_EINVAL;
   if (s->cfg.overlay_flags & ~NES_OVERLAY_ALL) return NES_EINVAL;
   // admission by memory: as many batch slots as the budget holds
-  const size_t per_slot = slot_bytes(s->cfg.max_batch_rays, 16);
+  const size_t per_slot = slot_bytes(s->cfg.max_batch_rays, s->cfg.precision);
   const uint64_t afford = s->cfg.hbm_budget_bytes / per_slot;
   if (afford < 1) {
     set_last_error("HBM budget below one batch slot");

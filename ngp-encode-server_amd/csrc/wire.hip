@@ -303,6 +303,13 @@ void serve_connection(nes_wire_server *s, int fd) {
     if (!write_full(fd, &n64, sizeof(n64)) || !write_full(fd, out.data(), n)) break;
     s->served++;
   }
+  {
+    // forget the descriptor before closing it, so nes_wire_server_stop never
+    // shuts down a number the process has since reused
+    std::lock_guard<std::mutex> lk(s->conn_mu);
+    for (int &f : s->fds)
+      if (f == fd) f = -1;
+  }
   ::close(fd);
 }
 
@@ -368,7 +375,8 @@ int nes_wire_server_stop(nes_wire_server *s) {
   if (s->acceptor.joinable()) s->acceptor.join();
   {
     std::lock_guard<std::mutex> lk(s->conn_mu);
-    for (int fd : s->fds) ::shutdown(fd, SHUT_RDWR);
+    for (int fd : s->fds)
+      if (fd >= 0) ::shutdown(fd, SHUT_RDWR);
   }
   for (auto &t : s->conns)
     if (t.joinable()) t.join();

@@ -146,3 +146,18 @@ def test_config2_stereo_1080p_subset(nes, oracle, gpu_ctx, scene1):
         print(f"stereo eye PSNR {p:.1f} dB, max depth err "
               f"{np.abs(o['depth'].reshape(-1)[pix] - depth).max():.2e}")
         assert p >= MIXED_PSNR_DB
+
+
+def test_context_budget_bounds_render_buffers(nes, scene1):
+    """A render whose buffers exceed the context's HBM budget fails with
+    NES_ENOMEM and leaves the context usable for renders that fit."""
+    t, w, o = scene1
+    with nes.Context(t, w, o, hbm_budget=200 << 20) as ctx:
+        small = [nes.camera(64, 48, nes.orbit_camera(30.0, 20.0))]
+        outs, _ = ctx.render(small)
+        big = [nes.camera(1920, 1080, nes.orbit_camera(30.0, 20.0, 2.0, e)) for e in (-0.032, 0.032)]
+        with pytest.raises(nes.NesError) as e:
+            ctx.render(big)
+        assert e.value.code == -12
+        again, _ = ctx.render(small)
+        assert np.array_equal(outs[0]["rgb24"], again[0]["rgb24"])

@@ -63,19 +63,32 @@ def test_poisson_clients_short_run(nes, gpu_ctx):
     assert st.mrays_per_s > 0
 
 
+def slot_bytes(rays, precision_fp32=False):
+    """Device bytes of one batch slot: per ray slot RayData 32 + RayAcc 32 +
+    counts/offsets 8 + outputs 21 + YUV 2; per sample (4 per ray slot, at
+    least 2^20) record 4 + position 16 + MLP output 16 + 8 feature planes
+    (8 B FP16, 16 B FP32); 1/8 buffer growth slack; 1 MiB."""
+    cap = max(4 * rays, 1 << 20)
+    per_sample = 4 + 16 + 16 + (16 if precision_fp32 else 8) * 8
+    return (rays * (32 + 32 + 8 + 21 + 2) + cap * per_sample) // 8 * 9 + (1 << 20)
+
+
 def test_hbm_budget_decides_batch_slots(nes, gpu_ctx):
     """Admission by memory: the number of concurrent batch slots is what the
     budget affords, and a budget below one slot is refused."""
     small_rays = 1 << 20
     with nes.Server(gpu_ctx, max_batch_rays=small_rays, max_inflight=8) as srv:
         assert srv.stats().slots == 8
-    # one slot of 2^20 ray slots needs about (64 + 31) B + 16 x 20 B per ray
-    per_slot = small_rays * (32 + 32 + 8 + 21 + 2 + 16 * 20) + (1 << 20)
+    per_slot = slot_bytes(small_rays)
     with nes.Server(gpu_ctx, max_batch_rays=small_rays, max_inflight=8,
                     hbm_budget=2 * per_slot + 1) as srv:
         assert srv.stats().slots == 2
         t = srv.submit(0, stereo(nes, 320, 240, 40.0))
         assert srv.wait(t)[0] == 0
+    with nes.Server(gpu_ctx, max_batch_rays=small_rays, max_inflight=8,
+                    hbm_budget=2 * slot_bytes(small_rays, True) + 1,
+                    precision=nes.PRECISION_FP32) as srv:
+        assert srv.stats().slots == 2
     with pytest.raises(nes.NesError) as e:
-        nes.Server(gpu_ctx, max_batch_rays=small_rays, hbm_budget=per_slot // 2)
+        nes.Server(gpu_ctx, max_batch_rays=small_rays, hbm_budget=per_slot - 1)
     assert e.value.code == -12
