@@ -311,71 +311,50 @@ struct OutPtrs {
   uint16_t *nsamp;
 };
 
-// One lane per ray slot.  A wavefront's rays (one 8x8 tile) own one
-// contiguous range of the sample list (march_emit reserved it with a single
-// atomic), so the wavefront stages that range through LDS in coalesced
-// chunks and each lane then composites its own segment, in order, from LDS.
-constexpr int kCompChunk = 256;
-
 __global__ __launch_bounds__(256) void composite(
     uint32_t n_slots, const RayData *__restrict__ rays, RayAcc *acc,
     const uint32_t *__restrict__ counts, const uint32_t *__restrict__ offsets,
     const uint32_t *__restrict__ samples, const float4 *__restrict__ sout,
     const uint32_t *__restrict__ n_ptr, uint32_t capacity, int keep, int last, OutPtrs o) {
-  __shared__ float4 s_out[4][kCompChunk];
-  __shared__ uint32_t s_k[4][kCompChunk];
-  if (*n_ptr > capacity) return;  // overflowed pass: the host re-renders
   const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
-  const int lane = lane_id(), wave = (int)(threadIdx.x >> 6);
-  const bool valid = slot < n_slots;
-  RayAcc a{};
-  uint32_t cnt = 0, off = 0;
-  if (valid) {
-    a = acc[slot];
-    cnt = counts[slot];
-    off = offsets[slot];
-  }
-  const bool live = valid && !a.done && cnt;
-  const float ts = live ? load_ray(rays, slot).t_start : 0.0f;
-  uint32_t lo = live ? off : 0xffffffffu, hi = live ? off + cnt : 0u;
+  if (slot >= n_slots) return;
+  if (*n_ptr > capacity) return;  // overflowed pass: the host re-renders
+  RayAcc a = acc[slot];
+  const uint32_t cnt = counts[slot];
+  if (!a.done && cnt) {
+    const uint32_t off = offsets[slot];
+    const float ts = load_ray(rays, slot).t_start;
+    // samples are fetched eight at a time (independent loads), composited in order
+    for (uint32_t j0 = 0; j0 < cnt && !a.done; j0 += 8) {
+      float4 sb[8];
+      uint32_t kb[8];
 #pragma unroll
-  for (int m = 32; m > 0; m >>= 1) {
-    lo = min(lo, (uint32_t)__shfl_xor((int)lo, m));
-    hi = max(hi, (uint32_t)__shfl_xor((int)hi, m));
-  }
-  for (uint32_t c0 = lo; c0 < hi; c0 += kCompChunk) {  // wavefront-uniform
-    const uint32_t cn = min((uint32_t)kCompChunk, hi - c0);
-    for (uint32_t i = (uint32_t)lane; i < cn; i += 64) {
-      s_out[wave][i] = sout[c0 + i];
-      s_k[wave][i] = samples[c0 + i];
-    }
-    wave_lds_fence();
-    if (live && !a.done) {
-      const uint32_t b = max(off, c0), e = min(off + cnt, c0 + cn);
-      for (uint32_t jj = b; jj < e; ++jj) {
-        const float4 sm = s_out[wave][jj - c0];
-        const float t = step_t((int)(s_k[wave][jj - c0] & 127u), ts);
-        const float sd = sm.x * kDelta;
+      for (int u = 0; u < 8; ++u) {
+        const uint32_t j = min(j0 + (uint32_t)u, cnt - 1);
+        sb[u] = sout[off + j];
+        kb[u] = samples[off + j];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        if (j0 + (uint32_t)u >= cnt || a.done) break;
+        const float4 s = sb[u];
+        const float t = step_t((int)(kb[u] & 127u), ts);
+        const float sd = s.x * kDelta;
         const float alpha = -expm1f(-sd);
         const float w = a.T * alpha;
-        a.r = fmaf(w, sm.y, a.r);
-        a.g = fmaf(w, sm.z, a.g);
-        a.b = fmaf(w, sm.w, a.b);
+        a.r = fmaf(w, s.y, a.r);
+        a.g = fmaf(w, s.z, a.g);
+        a.b = fmaf(w, s.w, a.b);
         a.depth = fmaf(w, t, a.depth);
         a.T *= expf(-sd);
         a.n_samples++;
-        if (a.T < kTStop) {
-          a.done = 1;
-          break;
-        }
+        if (a.T < kTStop) a.done = 1;
       }
     }
-    wave_lds_fence();  // every lane is done reading before the next chunk lands
+    // the state carries into the next pass; after the last pass only a
+    // caller that reads it again (keep: the training backward) stores it
+    if (!last || keep) acc[slot] = a;
   }
-  if (!valid) return;
-  // the state carries into the next pass; after the last pass only a caller
-  // that reads it again (keep: the training backward) stores it
-  if (live && (!last || keep)) acc[slot] = a;
   if (last && a.pad != 0xffffffffu) {
     const uint32_t p = a.pad;
     const float c[3] = {a.r, a.g, a.b};
