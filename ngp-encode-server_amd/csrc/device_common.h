@@ -93,6 +93,28 @@ __device__ __forceinline__ RayData load_ray(const RayData *rays, uint32_t slot) 
 
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
 
+// ------------------------------------------------- rank-major sample list
+// The samples of one wavefront's 64 ray slots (one 8x8 pixel tile) occupy
+// one contiguous range of the sample list, reserved with a single atomic.
+// Inside it they are ordered by rank: first the first occupied sample of
+// every ray that has one (in lane order), then every second sample, and so
+// on.  Rank i of lane l sits at
+//   base + sum_{r<i} |{lanes with count > r}| + |{lanes < l with count > i}|.
+// Neighbouring lanes of the encode and MLP kernels thus see samples of
+// neighbouring pixels at similar depths (their table gathers share cache
+// lines), and a ray's samples are still visited in march order.  Every
+// kernel that walks the list by ray runs one lane per slot with
+// slot = 64 * wavefront + lane and keeps all 64 lanes in the loop.
+__device__ __forceinline__ uint64_t lanes_below(int lane) {
+  return lane ? (~0ull >> (64 - lane)) : 0ull;
+}
+
+__device__ __forceinline__ uint32_t wave_max(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o));
+  return v;
+}
+
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);

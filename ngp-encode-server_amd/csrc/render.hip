@@ -33,11 +33,12 @@ namespace nes {
 
 // ----------------------------------------------------------- march kernel
 // One kernel for both march walks: count this ray's occupied steps, reserve
-// a contiguous segment of the sample list (wave scan + one atomic per
-// wavefront on a device-side cursor), then write the records.  The sample
-// total never leaves the device: later kernels read it from the cursor, and
-// writes beyond `capacity` are dropped (the host re-renders with a larger
-// buffer when the final total exceeds it).
+// the wavefront's range of the sample list (one atomic per wavefront on a
+// device-side cursor), then write the records in rank-major order.  The
+// sample total never leaves the device: later kernels read it from the
+// cursor, and writes beyond `capacity` are dropped (the host re-renders with a
+// larger buffer when the final total exceeds it).  counts[slot] receives the
+// ray's sample count, offsets[slot] the wavefront's range start.
 __global__ __launch_bounds__(256) void march_emit(
     const ViewDesc *__restrict__ views, uint32_t n_views, uint32_t n_slots,
     const uint32_t *__restrict__ occ, RayData *rays, RayAcc *acc, uint32_t *counts,
@@ -92,35 +93,38 @@ __global__ __launch_bounds__(256) void march_emit(
       }
     }
   }
-  uint32_t incl = cnt;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const uint32_t t = __shfl_up(incl, o);
-    if (lane >= o) incl += t;
-  }
-  const uint32_t wave_total = __shfl(incl, 63);
+  const uint32_t wave_total = wave_sum(cnt);
   uint32_t base = 0;
   if (lane == 0 && wave_total) base = atomicAdd(cursor, wave_total);
   base = __shfl(base, 0);
-  if (slot >= n_slots) return;
-  uint32_t off = base + incl - cnt;
-  counts[slot] = cnt;
-  offsets[slot] = off;
-#pragma unroll
-  for (int wd = 0; wd < 4; ++wd) {
-    uint32_t b = bits[wd];
-    while (b) {
+  if (slot < n_slots) {
+    counts[slot] = cnt;
+    offsets[slot] = base;
+  }
+  const uint32_t mx = wave_max(cnt);
+  const uint64_t below = lanes_below(lane);
+  uint32_t before = 0;  // samples of lower ranks
+  int wd = 0;
+  uint32_t b = bits[0];
+  for (uint32_t i = 0; i < mx; ++i) {  // wavefront-uniform
+    const uint64_t has = __ballot(cnt > i);
+    if (cnt > i) {
+      while (b == 0) {
+        ++wd;
+        b = wd == 1 ? bits[1] : wd == 2 ? bits[2] : bits[3];
+      }
       const int k = 32 * wd + __builtin_ctz(b);
       b &= b - 1;
-      if (off < capacity) {
+      const uint32_t at = base + before + (uint32_t)__popcll(has & below);
+      if (at < capacity) {
         float x[3], t;
         step_pos(r, k, x, &t);
-        pos[off] = make_float4(fminf(fmaxf(x[0], 0.0f), 1.0f), fminf(fmaxf(x[1], 0.0f), 1.0f),
-                               fminf(fmaxf(x[2], 0.0f), 1.0f), __uint_as_float(slot));
-        samples[off] = (slot << 7) | (uint32_t)k;
+        pos[at] = make_float4(fminf(fmaxf(x[0], 0.0f), 1.0f), fminf(fmaxf(x[1], 0.0f), 1.0f),
+                              fminf(fmaxf(x[2], 0.0f), 1.0f), __uint_as_float(slot));
+        samples[at] = (slot << 7) | (uint32_t)k;
       }
-      ++off;
     }
+    before += (uint32_t)__popcll(has);
   }
 }
 
@@ -311,50 +315,71 @@ struct OutPtrs {
   uint16_t *nsamp;
 };
 
+// One lane per ray slot, walking the wavefront's rank-major sample range
+// eight ranks at a time (positions from ballots, loads issued together, then
+// composited in order).
 __global__ __launch_bounds__(256) void composite(
     uint32_t n_slots, const RayData *__restrict__ rays, RayAcc *acc,
     const uint32_t *__restrict__ counts, const uint32_t *__restrict__ offsets,
     const uint32_t *__restrict__ samples, const float4 *__restrict__ sout,
     const uint32_t *__restrict__ n_ptr, uint32_t capacity, int keep, int last, OutPtrs o) {
-  const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
-  if (slot >= n_slots) return;
   if (*n_ptr > capacity) return;  // overflowed pass: the host re-renders
-  RayAcc a = acc[slot];
-  const uint32_t cnt = counts[slot];
-  if (!a.done && cnt) {
-    const uint32_t off = offsets[slot];
-    const float ts = load_ray(rays, slot).t_start;
-    // samples are fetched eight at a time (independent loads), composited in order
-    for (uint32_t j0 = 0; j0 < cnt && !a.done; j0 += 8) {
-      float4 sb[8];
-      uint32_t kb[8];
+  const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = lane_id();
+  const bool valid = slot < n_slots;
+  RayAcc a{};
+  uint32_t cnt = 0, base = 0;
+  if (valid) {
+    a = acc[slot];
+    cnt = counts[slot];
+    base = offsets[slot];
+  }
+  base = __shfl(base, 0);  // lane 0 is valid whenever any lane is
+  const bool live = valid && !a.done && cnt;
+  const float ts = live ? load_ray(rays, slot).t_start : 0.0f;
+  const uint32_t mx = wave_max(cnt);
+  const uint64_t below = lanes_below(lane);
+  uint32_t before = 0;
+  for (uint32_t i0 = 0; i0 < mx; i0 += 8) {  // wavefront-uniform
+    if (__ballot(live && !a.done && i0 < cnt) == 0) break;
+    float4 sb[8];
+    uint32_t kb[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const uint32_t j = min(j0 + (uint32_t)u, cnt - 1);
-        sb[u] = sout[off + j];
-        kb[u] = samples[off + j];
+    for (int u = 0; u < 8; ++u) {
+      const uint32_t i = i0 + (uint32_t)u;
+      const uint64_t has = __ballot(cnt > i);
+      const uint32_t at = base + before + (uint32_t)__popcll(has & below);
+      before += (uint32_t)__popcll(has);
+      sb[u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      kb[u] = 0u;
+      if (live && cnt > i && !a.done) {
+        sb[u] = sout[at];
+        kb[u] = samples[at];
       }
+    }
+    if (live) {
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
-        if (j0 + (uint32_t)u >= cnt || a.done) break;
-        const float4 s = sb[u];
+        if (i0 + (uint32_t)u >= cnt || a.done) break;
+        const float4 sm = sb[u];
         const float t = step_t((int)(kb[u] & 127u), ts);
-        const float sd = s.x * kDelta;
+        const float sd = sm.x * kDelta;
         const float alpha = -expm1f(-sd);
         const float w = a.T * alpha;
-        a.r = fmaf(w, s.y, a.r);
-        a.g = fmaf(w, s.z, a.g);
-        a.b = fmaf(w, s.w, a.b);
+        a.r = fmaf(w, sm.y, a.r);
+        a.g = fmaf(w, sm.z, a.g);
+        a.b = fmaf(w, sm.w, a.b);
         a.depth = fmaf(w, t, a.depth);
         a.T *= expf(-sd);
         a.n_samples++;
         if (a.T < kTStop) a.done = 1;
       }
     }
-    // the state carries into the next pass; after the last pass only a
-    // caller that reads it again (keep: the training backward) stores it
-    if (!last || keep) acc[slot] = a;
   }
+  if (!valid) return;
+  // the state carries into the next pass; after the last pass only a caller
+  // that reads it again (keep: the training backward) stores it
+  if (live && (!last || keep)) acc[slot] = a;
   if (last && a.pad != 0xffffffffu) {
     const uint32_t p = a.pad;
     const float c[3] = {a.r, a.g, a.b};

@@ -361,34 +361,59 @@ __global__ __launch_bounds__(256) void train_dcomposite(
     const uint32_t *__restrict__ counts, const uint32_t *__restrict__ offsets,
     const float4 *__restrict__ sout, const float *__restrict__ dC, float4 *__restrict__ dout) {
   const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
-  if (slot >= n_slots) return;
-  const uint32_t cnt = counts[slot];
-  if (!cnt) return;
-  const uint32_t off = offsets[slot];
-  const RayAcc a = acc[slot];
-  const uint32_t used = min(cnt, a.n_samples);
-  const float g[3] = {dC[3 * a.pad + 0], dC[3 * a.pad + 1], dC[3 * a.pad + 2]};
+  const int lane = lane_id();
+  const bool valid = slot < n_slots;
+  uint32_t cnt = 0, base = 0, used = 0;
+  float g[3] = {0.0f, 0.0f, 0.0f};
+  if (valid) {
+    cnt = counts[slot];
+    base = offsets[slot];
+    if (cnt) {
+      const RayAcc a = acc[slot];
+      used = min(cnt, a.n_samples);
+      g[0] = dC[3 * a.pad + 0];
+      g[1] = dC[3 * a.pad + 1];
+      g[2] = dC[3 * a.pad + 2];
+    }
+  }
+  base = __shfl(base, 0);  // lane 0 is valid whenever any lane is
+  const uint32_t mx = wave_max(cnt);
+  const uint64_t below = lanes_below(lane);
+  // front to back over the ray's samples (rank-major positions)
   float T = 1.0f;
-  for (uint32_t k = 0; k < used; ++k) {
-    const float4 s = sout[off + k];
-    const float sd = s.x * kDelta;
-    const float w = T * -expm1f(-sd);
-    const float Tn = T * expf(-sd);
-    const float cg = s.y * g[0] + s.z * g[1] + s.w * g[2];
-    dout[off + k] = make_float4(Tn * cg, w * cg, w, 0.0f);
-    T = Tn;
+  uint32_t before = 0;
+  for (uint32_t i = 0; i < mx; ++i) {  // wavefront-uniform
+    const uint64_t has = __ballot(cnt > i);
+    const uint32_t at = base + before + (uint32_t)__popcll(has & below);
+    before += (uint32_t)__popcll(has);
+    if (i < used) {
+      const float4 s = sout[at];
+      const float sd = s.x * kDelta;
+      const float w = T * -expm1f(-sd);
+      const float Tn = T * expf(-sd);
+      const float cg = s.y * g[0] + s.z * g[1] + s.w * g[2];
+      dout[at] = make_float4(Tn * cg, w * cg, w, 0.0f);
+      T = Tn;
+    } else if (i < cnt) {
+      dout[at] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);  // after termination
+    }
   }
+  // back to front: suffix sums S_i and the final gradients
   float S = 0.0f;
-  for (uint32_t k = used; k-- > 0;) {
-    const float4 p = dout[off + k];
-    const float4 s = sout[off + k];
-    const float dsigma = kDelta * (p.x - S);
-    S += p.y;
-    const float w = p.z;
-    dout[off + k] = make_float4(dsigma * s.x, w * g[0] * s.y * (1.0f - s.y),
-                                w * g[1] * s.z * (1.0f - s.z), w * g[2] * s.w * (1.0f - s.w));
+  for (uint32_t i = mx; i-- > 0;) {
+    const uint64_t has = __ballot(cnt > i);
+    before -= (uint32_t)__popcll(has);
+    const uint32_t at = base + before + (uint32_t)__popcll(has & below);
+    if (i < used) {
+      const float4 p = dout[at];
+      const float4 s = sout[at];
+      const float dsigma = kDelta * (p.x - S);
+      S += p.y;
+      const float w = p.z;
+      dout[at] = make_float4(dsigma * s.x, w * g[0] * s.y * (1.0f - s.y),
+                             w * g[1] * s.z * (1.0f - s.z), w * g[2] * s.w * (1.0f - s.w));
+    }
   }
-  for (uint32_t k = used; k < cnt; ++k) dout[off + k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   (void)rays;
 }
 
