@@ -249,6 +249,37 @@ __device__ __forceinline__ float2 encode_one(const Tab *__restrict__ table, cons
   return make_float2(a0, a1);
 }
 
+// Sample positions as the encode reads them.  The FP32 path keeps the exact
+// float4 (x, y, z, slot).  The mixed path rounds x, y, z in [0, 1] to
+// multiples of 2^-21 and packs them 21 bits each into 8 bytes: every XCD part
+// reads every position, so this halves the encode's position stream; the
+// rounding (at most 2^-22) moves a level-15 interpolation weight by at most
+// 2048 * 2^-22 = 4.9e-4, below the FP16 table's own rounding.
+__device__ __forceinline__ uint2 pack_pos21(const float x[3]) {
+  uint32_t q[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) q[a] = min(__float2uint_rn(x[a] * 2097152.0f), 2097151u);
+  return make_uint2(q[0] | (q[1] << 21), (q[1] >> 11) | (q[2] << 10));
+}
+
+template <class Tab> struct PosIn;
+template <> struct PosIn<float> {
+  typedef float4 type;
+  static __device__ __forceinline__ void get(const float4 &p, float x[3]) {
+    x[0] = p.x;
+    x[1] = p.y;
+    x[2] = p.z;
+  }
+};
+template <> struct PosIn<_Float16> {
+  typedef uint2 type;
+  static __device__ __forceinline__ void get(const uint2 &p, float x[3]) {
+    x[0] = (float)(p.x & 0x1FFFFFu) * 0x1p-21f;
+    x[1] = (float)((p.x >> 21) | ((p.y & 0x3FFu) << 11)) * 0x1p-21f;
+    x[2] = (float)(p.y >> 10) * 0x1p-21f;
+  }
+};
+
 template <class Tab> struct PartOut;
 template <> struct PartOut<float> {
   typedef float4 type;

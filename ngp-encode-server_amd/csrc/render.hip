@@ -43,7 +43,7 @@ __global__ __launch_bounds__(256) void march_emit(
     const ViewDesc *__restrict__ views, uint32_t n_views, uint32_t n_slots,
     const uint32_t *__restrict__ occ, RayData *rays, RayAcc *acc, uint32_t *counts,
     uint32_t *offsets, uint32_t *samples, float4 *__restrict__ pos, uint32_t *cursor,
-    uint32_t capacity, int k_begin, int k_end, int first) {
+    uint32_t capacity, int k_begin, int k_end, int first, int packed) {
   const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = lane_id();
   uint32_t cnt = 0;
@@ -119,8 +119,12 @@ __global__ __launch_bounds__(256) void march_emit(
       if (at < capacity) {
         float x[3], t;
         step_pos(r, k, x, &t);
-        pos[at] = make_float4(fminf(fmaxf(x[0], 0.0f), 1.0f), fminf(fmaxf(x[1], 0.0f), 1.0f),
-                              fminf(fmaxf(x[2], 0.0f), 1.0f), __uint_as_float(slot));
+        const float xc[3] = {fminf(fmaxf(x[0], 0.0f), 1.0f), fminf(fmaxf(x[1], 0.0f), 1.0f),
+                             fminf(fmaxf(x[2], 0.0f), 1.0f)};
+        if (packed)
+          reinterpret_cast<uint2 *>(pos)[at] = pack_pos21(xc);
+        else
+          pos[at] = make_float4(xc[0], xc[1], xc[2], __uint_as_float(slot));
         samples[at] = (slot << 7) | (uint32_t)k;
       }
     }
@@ -190,7 +194,7 @@ __device__ __forceinline__ half4 sh4_quarter(const float d[3], int g) {
 // Infinity Cache.  (In FP32 a hashed level alone is 4 MB.)  Features go to
 // feat[p][sample] as {level p f0, f1, level 15-p f0, f1}.
 template <class Tab>
-__global__ __launch_bounds__(256) void encode_parts(const float4 *__restrict__ pos,
+__global__ __launch_bounds__(256) void encode_parts(const typename PosIn<Tab>::type *__restrict__ pos,
                                                     const uint32_t *__restrict__ n_ptr,
                                                     uint32_t stride, const Tab *__restrict__ table,
                                                     typename PartOut<Tab>::type *__restrict__ feat,
@@ -201,8 +205,8 @@ __global__ __launch_bounds__(256) void encode_parts(const float4 *__restrict__ p
   const PartLevels lp = part_levels(part);
   typename PartOut<Tab>::type *dst = feat + (size_t)part * stride;
   for (uint32_t s = grp * blockDim.x + threadIdx.x; s < n; s += blocks_per_part * blockDim.x) {
-    const float4 p = pos[s];
-    const float xc[3] = {p.x, p.y, p.z};
+    float xc[3];
+    PosIn<Tab>::get(pos[s], xc);
     const float2 fa = encode_one(table, lp, 0, xc);
     const float2 fb = encode_one(table, lp, 1, xc);
     dst[s] = PartOut<Tab>::pack(fa, fb);
@@ -572,7 +576,8 @@ int render_launch_pass(const SceneDev &scene, RenderWork &w, int precision, int 
                                         w.rays.as<RayData>(), w.acc.as<RayAcc>(),
                                         w.counts.as<uint32_t>(), w.offsets.as<uint32_t>(),
                                         w.samples.as<uint32_t>(), w.sample_pos.as<float4>(),
-                                        cursor, cap, k0, k1, first ? 1 : 0);
+                                        cursor, cap, k0, k1, first ? 1 : 0,
+                                        precision == NES_PRECISION_MIXED ? 1 : 0);
   NES_HIP_TRY(hipGetLastError());
   if (ev_eval_begin) NES_HIP_TRY(hipEventRecord(ev_eval_begin, stream));
   // fixed grids with grid-stride loops: the sample count stays on the device
@@ -587,7 +592,7 @@ int render_launch_pass(const SceneDev &scene, RenderWork &w, int precision, int 
                                           cap, w.rays.as<RayData>(), scene.weights_f32,
                                           w.sample_out.as<float4>());
   } else {
-    encode_parts<_Float16><<<bpp * kParts, 256, 0, stream>>>(w.sample_pos.as<float4>(), cursor,
+    encode_parts<_Float16><<<bpp * kParts, 256, 0, stream>>>(w.sample_pos.as<uint2>(), cursor,
                                                               cap, scene.table_f16,
                                                               w.feat.as<uint2>(), bpp);
     NES_HIP_TRY(hipGetLastError());
@@ -618,7 +623,7 @@ int launch_march_emit(const uint32_t *occ, RenderWork &w, int k0, int k1, hipStr
                                         w.rays.as<RayData>(), w.acc.as<RayAcc>(),
                                         w.counts.as<uint32_t>(), w.offsets.as<uint32_t>(),
                                         w.samples.as<uint32_t>(), w.sample_pos.as<float4>(),
-                                        cursor, w.capacity, k0, k1, 0);
+                                        cursor, w.capacity, k0, k1, 0, 0);
   NES_HIP_TRY(hipGetLastError());
   return NES_OK;
 }
