@@ -43,7 +43,7 @@ __global__ __launch_bounds__(256) void march_emit(
     const ViewDesc *__restrict__ views, uint32_t n_views, uint32_t n_slots,
     const uint32_t *__restrict__ occ, RayData *rays, RayAcc *acc, uint32_t *counts,
     uint32_t *offsets, uint32_t *samples, float4 *__restrict__ pos, uint32_t *cursor,
-    uint32_t capacity, int k_begin, int k_end, int first, int packed) {
+    uint32_t capacity, int k_begin, int k_end, int first, int packed, uint2 *ray_sh) {
   const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = lane_id();
   uint32_t cnt = 0;
@@ -77,6 +77,22 @@ __global__ __launch_bounds__(256) void march_emit(
       rays[slot] = r;
       acc[slot] = a;
       active = !a.done;
+      if (ray_sh) {
+        // the colour MLP's SH input of this ray, in FP16 (mixed path)
+        float sh[16];
+        sh4(r.d, sh);
+        uint32_t hw[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          half2v h;
+          h[0] = (_Float16)sh[2 * k];
+          h[1] = (_Float16)sh[2 * k + 1];
+          hw[k] = __builtin_bit_cast(uint32_t, h);
+        }
+        uint4 *dst = reinterpret_cast<uint4 *>(ray_sh + (size_t)slot * 4);
+        dst[0] = make_uint4(hw[0], hw[1], hw[2], hw[3]);
+        dst[1] = make_uint4(hw[4], hw[5], hw[6], hw[7]);
+      }
     } else {
       r = load_ray(rays, slot);
       active = !acc[slot].done;
@@ -161,29 +177,27 @@ __device__ __forceinline__ float4v mfma(half8 a, half8 b, float4v c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
 }
 
+// ReLU of eight FP32 accumulators as the next layer's FP16 B operand: round
+// pairs to FP16 (v_cvt_pk_f16_f32, round to nearest even) and clamp at zero
+// in FP16 (v_pk_max_f16); rounding is monotonic and keeps the sign, so this
+// equals rounding the clamped values.
 __device__ __forceinline__ half8 pack_relu(float4v lo, float4v hi) {
+  typedef float float2v_ __attribute__((ext_vector_type(2)));
+  const half2v z = {(_Float16)0.0f, (_Float16)0.0f};
+  const half2v a = __builtin_elementwise_max(__builtin_convertvector((float2v_){lo[0], lo[1]}, half2v), z);
+  const half2v b = __builtin_elementwise_max(__builtin_convertvector((float2v_){lo[2], lo[3]}, half2v), z);
+  const half2v c = __builtin_elementwise_max(__builtin_convertvector((float2v_){hi[0], hi[1]}, half2v), z);
+  const half2v d = __builtin_elementwise_max(__builtin_convertvector((float2v_){hi[2], hi[3]}, half2v), z);
   half8 r;
-  r[0] = (_Float16)fmaxf(lo[0], 0.0f); r[1] = (_Float16)fmaxf(lo[1], 0.0f);
-  r[2] = (_Float16)fmaxf(lo[2], 0.0f); r[3] = (_Float16)fmaxf(lo[3], 0.0f);
-  r[4] = (_Float16)fmaxf(hi[0], 0.0f); r[5] = (_Float16)fmaxf(hi[1], 0.0f);
-  r[6] = (_Float16)fmaxf(hi[2], 0.0f); r[7] = (_Float16)fmaxf(hi[3], 0.0f);
+  r[0] = a[0]; r[1] = a[1]; r[2] = b[0]; r[3] = b[1];
+  r[4] = c[0]; r[5] = c[1]; r[6] = d[0]; r[7] = d[1];
   return r;
 }
 
-// SH components 4g .. 4g+3 of a unit direction.
-__device__ __forceinline__ half4 sh4_quarter(const float d[3], int g) {
-  float sh[16];
-  sh4(d, sh);
-  half4 r;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    float v = sh[k];
-#pragma unroll
-    for (int c = 1; c < 4; ++c)
-      if (g == c) v = sh[4 * c + k];
-    r[k] = (_Float16)v;
-  }
-  return r;
+// Mixed-path sigmoid: hardware exp and reciprocal (about 1 ulp each, far
+// below the FP16 network's own rounding).
+__device__ __forceinline__ float sigmoid_fast(float v) {
+  return __builtin_amdgcn_rcpf(1.0f + __expf(-v));
 }
 
 // ------------------------------------------------- XCD-partitioned encode
@@ -257,7 +271,7 @@ __global__ __launch_bounds__(256) void mlp_mixed(const uint32_t *__restrict__ sa
                                                  const uint2 *__restrict__ feat,
                                                  const uint32_t *__restrict__ n_ptr,
                                                  uint32_t stride,
-                                                 const RayData *__restrict__ rays,
+                                                 const uint2 *__restrict__ ray_sh,
                                                  const half8 *__restrict__ wfrag,
                                                  float4 *__restrict__ out) {
   const uint32_t n = min(*n_ptr, stride);
@@ -275,7 +289,8 @@ __global__ __launch_bounds__(256) void mlp_mixed(const uint32_t *__restrict__ sa
     const uint32_t sc = min(s, n - 1);  // n >= 1 inside the loop
     const uint2 fa = feat[(size_t)(2 * g) * stride + sc];
     const uint2 fb = feat[(size_t)(2 * g + 1) * stride + sc];
-    const RayData r = load_ray(rays, samples[sc] >> 7);  // record = slot << 7 | step
+    // SH components 4g .. 4g+3 of the sample's ray (record = slot << 7 | step)
+    const uint2 shw = ray_sh[(size_t)(samples[sc] >> 7) * 4 + g];
     const uint32_t words[4] = {fa.x, fa.y, fb.x, fb.y};
     half8 b1;
 #pragma unroll
@@ -289,11 +304,11 @@ __global__ __launch_bounds__(256) void mlp_mixed(const uint32_t *__restrict__ sa
     for (int m = 0; m < 4; ++m) d1[m] = mfma(wf[64 * m], b1, zero);
     float4v d2 = mfma(wf[64 * 4], pack_relu(d1[0], d1[1]), zero);
     d2 = mfma(wf[64 * 5], pack_relu(d1[2], d1[3]), d2);
-    const half4 shv = sh4_quarter(r.d, g);
+    const half2v sh01 = __builtin_bit_cast(half2v, shw.x), sh23 = __builtin_bit_cast(half2v, shw.y);
     half8 b3;
     b3[0] = (_Float16)d2[0]; b3[1] = (_Float16)d2[1];
     b3[2] = (_Float16)d2[2]; b3[3] = (_Float16)d2[3];
-    b3[4] = shv[0]; b3[5] = shv[1]; b3[6] = shv[2]; b3[7] = shv[3];
+    b3[4] = sh01[0]; b3[5] = sh01[1]; b3[6] = sh23[0]; b3[7] = sh23[1];
     float4v d3[4];
 #pragma unroll
     for (int m = 0; m < 4; ++m) d3[m] = mfma(wf[64 * (6 + m)], b3, zero);
@@ -307,7 +322,8 @@ __global__ __launch_bounds__(256) void mlp_mixed(const uint32_t *__restrict__ sa
     float4v d5 = mfma(wf[64 * 18], pack_relu(d4[0], d4[1]), zero);
     d5 = mfma(wf[64 * 19], pack_relu(d4[2], d4[3]), d5);
     if (g == 0 && s < n)
-      out[s] = make_float4(expf(d2[0]), sigmoidf_(d5[0]), sigmoidf_(d5[1]), sigmoidf_(d5[2]));
+      out[s] = make_float4(__expf(d2[0]), sigmoid_fast(d5[0]), sigmoid_fast(d5[1]),
+                           sigmoid_fast(d5[2]));
   }
 }
 
@@ -471,7 +487,7 @@ void scene_free(SceneDev &s) {
 }
 
 void render_work_free(RenderWork &w) {
-  for (DevBuf *b : {&w.views, &w.rays, &w.acc, &w.counts, &w.offsets, &w.samples,
+  for (DevBuf *b : {&w.views, &w.rays, &w.acc, &w.counts, &w.offsets, &w.ray_sh, &w.samples,
                     &w.sample_out, &w.sample_pos, &w.feat, &w.out, &w.total})
     b->release();
   if (w.total_host) hipHostFree(w.total_host);
@@ -543,6 +559,7 @@ int render_setup(RenderWork &w, const nes_camera *views, int n_views, hipStream_
   NES_TRY(w.acc.reserve(sizeof(RayAcc) * slots));
   NES_TRY(w.counts.reserve(sizeof(uint32_t) * slots));
   NES_TRY(w.offsets.reserve(sizeof(uint32_t) * slots));
+  NES_TRY(w.ray_sh.reserve(32 * slots));
   NES_TRY(w.out.reserve(out_layout(px).total));
   NES_TRY(w.total.reserve(64));
   if (!w.total_host) NES_HIP_TRY(hipHostMalloc((void **)&w.total_host, 64));
@@ -577,7 +594,9 @@ int render_launch_pass(const SceneDev &scene, RenderWork &w, int precision, int 
                                         w.counts.as<uint32_t>(), w.offsets.as<uint32_t>(),
                                         w.samples.as<uint32_t>(), w.sample_pos.as<float4>(),
                                         cursor, cap, k0, k1, first ? 1 : 0,
-                                        precision == NES_PRECISION_MIXED ? 1 : 0);
+                                        precision == NES_PRECISION_MIXED ? 1 : 0,
+                                        precision == NES_PRECISION_MIXED ? w.ray_sh.as<uint2>()
+                                                                         : nullptr);
   NES_HIP_TRY(hipGetLastError());
   if (ev_eval_begin) NES_HIP_TRY(hipEventRecord(ev_eval_begin, stream));
   // fixed grids with grid-stride loops: the sample count stays on the device
@@ -598,7 +617,7 @@ int render_launch_pass(const SceneDev &scene, RenderWork &w, int precision, int 
     NES_HIP_TRY(hipGetLastError());
     if (ev_encode_end) NES_HIP_TRY(hipEventRecord(ev_encode_end, stream));
     mlp_mixed<<<256 * 8, 256, 0, stream>>>(w.samples.as<uint32_t>(), w.feat.as<uint2>(), cursor,
-                                           cap, w.rays.as<RayData>(),
+                                           cap, w.ray_sh.as<uint2>(),
                                            reinterpret_cast<const half8 *>(scene.wfrag),
                                            w.sample_out.as<float4>());
   }
@@ -623,7 +642,7 @@ int launch_march_emit(const uint32_t *occ, RenderWork &w, int k0, int k1, hipStr
                                         w.rays.as<RayData>(), w.acc.as<RayAcc>(),
                                         w.counts.as<uint32_t>(), w.offsets.as<uint32_t>(),
                                         w.samples.as<uint32_t>(), w.sample_pos.as<float4>(),
-                                        cursor, w.capacity, k0, k1, 0, 0);
+                                        cursor, w.capacity, k0, k1, 0, 0, nullptr);
   NES_HIP_TRY(hipGetLastError());
   return NES_OK;
 }
@@ -659,7 +678,7 @@ int render_prepare_capacity(RenderWork &w, int precision) {
 
 size_t work_device_bytes(const RenderWork &w) {
   size_t b = 0;
-  for (const DevBuf *d : {&w.views, &w.rays, &w.acc, &w.counts, &w.offsets, &w.samples,
+  for (const DevBuf *d : {&w.views, &w.rays, &w.acc, &w.counts, &w.offsets, &w.ray_sh, &w.samples,
                           &w.sample_out, &w.sample_pos, &w.feat, &w.out, &w.total})
     b += d->bytes;
   return b;
@@ -669,7 +688,7 @@ size_t work_device_bytes(const RenderWork &w) {
 // render starts from nothing, and report NES_ENOMEM.
 static int check_budget(RenderWork &w) {
   if (w.budget && work_device_bytes(w) > w.budget) {
-    for (DevBuf *d : {&w.views, &w.rays, &w.acc, &w.counts, &w.offsets, &w.samples,
+    for (DevBuf *d : {&w.views, &w.rays, &w.acc, &w.counts, &w.offsets, &w.ray_sh, &w.samples,
                       &w.sample_out, &w.sample_pos, &w.feat, &w.out, &w.total})
       d->release();
     w.capacity = 0;
