@@ -205,15 +205,40 @@ __device__ __forceinline__ PartLevels part_levels(int part) {
   return p;
 }
 
+// Constants of one hash-grid level.
+struct LevelInfo {
+  float scale;
+  uint32_t n1, n1sq, off;
+  bool dense;
+};
+
+__device__ __forceinline__ LevelInfo level_info(int l) {
+  LevelInfo v;
+  int res = 0;
+  uint32_t off = 0;
+#pragma unroll
+  for (int c = 0; c < 16; ++c)
+    if (l == c) {
+      res = kLevelRes[c];
+      off = kLevelOffset[c];
+    }
+  v.scale = (float)res;
+  v.n1 = (uint32_t)res + 1u;
+  v.n1sq = v.n1 * v.n1;
+  v.off = off;
+  v.dense = l < kDenseLevels;
+  return v;
+}
+
 template <class Tab>
-__device__ __forceinline__ float2 encode_one(const Tab *__restrict__ table, const PartLevels &lp,
-                                             int q, const float xc[3]) {
+__device__ __forceinline__ float2 encode_level(const Tab *__restrict__ table, const LevelInfo &lv,
+                                               const float xc[3]) {
   uint32_t i0[3];
   float fr[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const float pp = xc[a] * lp.scale[q];
-    const float fl = fminf(floorf(pp), lp.scale[q] - 1.0f);
+    const float pp = xc[a] * lv.scale;
+    const float fl = fminf(floorf(pp), lv.scale - 1.0f);
     i0[a] = (uint32_t)fl;
     fr[a] = pp - fl;
   }
@@ -224,9 +249,9 @@ __device__ __forceinline__ float2 encode_one(const Tab *__restrict__ table, cons
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const uint32_t x = i0[0] + (uint32_t)h, y = i0[1] + ((c >> 1) & 1), z = i0[2] + ((c >> 2) & 1);
-      const uint32_t dense = x + y * lp.n1[q] + z * lp.n1sq[q];
+      const uint32_t dense = x + y * lv.n1 + z * lv.n1sq;
       const uint32_t hash = (x ^ (y * kPrime1) ^ (z * kPrime2)) & ((1u << kLog2T) - 1u);
-      e[h] = lp.off[q] + (lp.dense[q] ? dense : hash);
+      e[h] = lv.off + (lv.dense ? dense : hash);
     }
     // one 4-byte (FP16) or 8-byte (FP32) gather per corner: fetching the
     // (x, x+1) pair as one wider load when it shares an aligned word costs a
@@ -247,6 +272,18 @@ __device__ __forceinline__ float2 encode_one(const Tab *__restrict__ table, cons
     a1 = fmaf(w, fv.y, a1);
   }
   return make_float2(a0, a1);
+}
+
+template <class Tab>
+__device__ __forceinline__ float2 encode_one(const Tab *__restrict__ table, const PartLevels &lp,
+                                             int q, const float xc[3]) {
+  LevelInfo lv;
+  lv.scale = lp.scale[q];
+  lv.n1 = lp.n1[q];
+  lv.n1sq = lp.n1sq[q];
+  lv.off = lp.off[q];
+  lv.dense = lp.dense[q];
+  return encode_level(table, lv, xc);
 }
 
 // Sample positions as the encode reads them.  The FP32 path keeps the exact

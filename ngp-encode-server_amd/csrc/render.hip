@@ -77,22 +77,6 @@ __global__ __launch_bounds__(256) void march_emit(
       rays[slot] = r;
       acc[slot] = a;
       active = !a.done;
-      if (ray_sh) {
-        // the colour MLP's SH input of this ray, in FP16 (mixed path)
-        float sh[16];
-        sh4(r.d, sh);
-        uint32_t hw[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-          half2v h;
-          h[0] = (_Float16)sh[2 * k];
-          h[1] = (_Float16)sh[2 * k + 1];
-          hw[k] = __builtin_bit_cast(uint32_t, h);
-        }
-        uint4 *dst = reinterpret_cast<uint4 *>(ray_sh + (size_t)slot * 4);
-        dst[0] = make_uint4(hw[0], hw[1], hw[2], hw[3]);
-        dst[1] = make_uint4(hw[4], hw[5], hw[6], hw[7]);
-      }
     } else {
       r = load_ray(rays, slot);
       active = !acc[slot].done;
@@ -108,6 +92,23 @@ __global__ __launch_bounds__(256) void march_emit(
         }
       }
     }
+  }
+  if (ray_sh && cnt) {
+    // the colour MLP's SH input of this ray, in FP16 (mixed path); written by
+    // every pass in which the ray has samples
+    float sh[16];
+    sh4(r.d, sh);
+    uint32_t hw[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      half2v h;
+      h[0] = (_Float16)sh[2 * k];
+      h[1] = (_Float16)sh[2 * k + 1];
+      hw[k] = __builtin_bit_cast(uint32_t, h);
+    }
+    uint4 *dst = reinterpret_cast<uint4 *>(ray_sh + (size_t)slot * 4);
+    dst[0] = make_uint4(hw[0], hw[1], hw[2], hw[3]);
+    dst[1] = make_uint4(hw[4], hw[5], hw[6], hw[7]);
   }
   const uint32_t wave_total = wave_sum(cnt);
   uint32_t base = 0;
