@@ -36,6 +36,7 @@ constexpr uint32_t kMaxSlots = 1u << 25;
 struct RenderWork {
   DevBuf views, rays, acc, counts, offsets, samples, sample_out, sample_pos, feat, out, total;
   DevBuf ray_sh;                  // mixed path: per ray slot 16 FP16 SH values
+  DevBuf tinfo;                   // single-pass renders: per ray slot t_start, pixel
   uint32_t *total_host = nullptr;  // pinned
   hipEvent_t ev[8] = {};
   uint32_t n_views = 0, n_slots = 0;

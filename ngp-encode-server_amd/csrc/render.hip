@@ -43,7 +43,8 @@ __global__ __launch_bounds__(256) void march_emit(
     const ViewDesc *__restrict__ views, uint32_t n_views, uint32_t n_slots,
     const uint32_t *__restrict__ occ, RayData *rays, RayAcc *acc, uint32_t *counts,
     uint32_t *offsets, uint32_t *samples, float4 *__restrict__ pos, uint32_t *cursor,
-    uint32_t capacity, int k_begin, int k_end, int first, int packed, uint2 *ray_sh) {
+    uint32_t capacity, int k_begin, int k_end, int first, int packed, uint2 *ray_sh,
+    uint2 *tinfo) {
   const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = lane_id();
   uint32_t cnt = 0;
@@ -74,8 +75,14 @@ __global__ __launch_bounds__(256) void march_emit(
         a.pad = 0xffffffffu;
       }
       a.done = r.n_steps == 0;
-      rays[slot] = r;
-      acc[slot] = a;
+      if (tinfo) {
+        // single-pass render: compositing needs only t_start and the pixel
+        tinfo[slot] = make_uint2(__float_as_uint(r.t_start), a.pad);
+      } else {
+        acc[slot] = a;
+      }
+      // the FP32 MLP reads the ray direction; the mixed one reads ray_sh
+      if (!tinfo || !packed) rays[slot] = r;
       active = !a.done;
     } else {
       r = load_ray(rays, slot);
@@ -343,21 +350,31 @@ __global__ __launch_bounds__(256) void composite(
     uint32_t n_slots, const RayData *__restrict__ rays, RayAcc *acc,
     const uint32_t *__restrict__ counts, const uint32_t *__restrict__ offsets,
     const uint32_t *__restrict__ samples, const float4 *__restrict__ sout,
-    const uint32_t *__restrict__ n_ptr, uint32_t capacity, int keep, int last, OutPtrs o) {
+    const uint32_t *__restrict__ n_ptr, uint32_t capacity, int keep, int last, OutPtrs o,
+    const uint2 *__restrict__ tinfo) {
   if (*n_ptr > capacity) return;  // overflowed pass: the host re-renders
   const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = lane_id();
   const bool valid = slot < n_slots;
   RayAcc a{};
   uint32_t cnt = 0, base = 0;
+  float ts = 0.0f;
   if (valid) {
-    a = acc[slot];
+    if (tinfo) {
+      // single-pass render: fresh state, t_start and pixel from the march
+      const uint2 ti = tinfo[slot];
+      a.T = 1.0f;
+      a.pad = ti.y;
+      ts = __uint_as_float(ti.x);
+    } else {
+      a = acc[slot];
+    }
     cnt = counts[slot];
     base = offsets[slot];
   }
   base = __shfl(base, 0);  // lane 0 is valid whenever any lane is
   const bool live = valid && !a.done && cnt;
-  const float ts = live ? load_ray(rays, slot).t_start : 0.0f;
+  if (live && !tinfo) ts = load_ray(rays, slot).t_start;
   const uint32_t mx = wave_max(cnt);
   const uint64_t below = lanes_below(lane);
   uint32_t before = 0;
@@ -488,7 +505,7 @@ void scene_free(SceneDev &s) {
 }
 
 void render_work_free(RenderWork &w) {
-  for (DevBuf *b : {&w.views, &w.rays, &w.acc, &w.counts, &w.offsets, &w.ray_sh, &w.samples,
+  for (DevBuf *b : {&w.views, &w.rays, &w.acc, &w.counts, &w.offsets, &w.ray_sh, &w.tinfo, &w.samples,
                     &w.sample_out, &w.sample_pos, &w.feat, &w.out, &w.total})
     b->release();
   if (w.total_host) hipHostFree(w.total_host);
@@ -561,6 +578,7 @@ int render_setup(RenderWork &w, const nes_camera *views, int n_views, hipStream_
   NES_TRY(w.counts.reserve(sizeof(uint32_t) * slots));
   NES_TRY(w.offsets.reserve(sizeof(uint32_t) * slots));
   NES_TRY(w.ray_sh.reserve(32 * slots));
+  NES_TRY(w.tinfo.reserve(8 * slots));
   NES_TRY(w.out.reserve(out_layout(px).total));
   NES_TRY(w.total.reserve(64));
   if (!w.total_host) NES_HIP_TRY(hipHostMalloc((void **)&w.total_host, 64));
@@ -589,6 +607,8 @@ int render_launch_pass(const SceneDev &scene, RenderWork &w, int precision, int 
   const unsigned blocks = (unsigned)((w.n_slots + 255) / 256);
   uint32_t *cursor = w.total.as<uint32_t>();
   const uint32_t cap = w.capacity;
+  // a render in one pass keeps no per-ray state between march and composite
+  uint2 *tinfo = first && last ? w.tinfo.as<uint2>() : nullptr;
   zero_u32<<<1, 64, 0, stream>>>(cursor, 2);
   march_emit<<<blocks, 256, 0, stream>>>(w.views.as<ViewDesc>(), w.n_views, w.n_slots, scene.occ,
                                         w.rays.as<RayData>(), w.acc.as<RayAcc>(),
@@ -597,7 +617,8 @@ int render_launch_pass(const SceneDev &scene, RenderWork &w, int precision, int 
                                         cursor, cap, k0, k1, first ? 1 : 0,
                                         precision == NES_PRECISION_MIXED ? 1 : 0,
                                         precision == NES_PRECISION_MIXED ? w.ray_sh.as<uint2>()
-                                                                         : nullptr);
+                                                                         : nullptr,
+                                        tinfo);
   NES_HIP_TRY(hipGetLastError());
   if (ev_eval_begin) NES_HIP_TRY(hipEventRecord(ev_eval_begin, stream));
   // fixed grids with grid-stride loops: the sample count stays on the device
@@ -627,7 +648,7 @@ int render_launch_pass(const SceneDev &scene, RenderWork &w, int precision, int 
   composite<<<blocks, 256, 0, stream>>>(w.n_slots, w.rays.as<RayData>(), w.acc.as<RayAcc>(),
                                        w.counts.as<uint32_t>(), w.offsets.as<uint32_t>(),
                                        w.samples.as<uint32_t>(), w.sample_out.as<float4>(),
-                                       cursor, cap, 0, last ? 1 : 0, out_ptrs(w));
+                                       cursor, cap, 0, last ? 1 : 0, out_ptrs(w), tinfo);
   NES_HIP_TRY(hipGetLastError());
   NES_HIP_TRY(hipMemcpyAsync(w.total_host, cursor, 4, hipMemcpyDeviceToHost, stream));
   return NES_OK;
@@ -643,7 +664,7 @@ int launch_march_emit(const uint32_t *occ, RenderWork &w, int k0, int k1, hipStr
                                         w.rays.as<RayData>(), w.acc.as<RayAcc>(),
                                         w.counts.as<uint32_t>(), w.offsets.as<uint32_t>(),
                                         w.samples.as<uint32_t>(), w.sample_pos.as<float4>(),
-                                        cursor, w.capacity, k0, k1, 0, 0, nullptr);
+                                        cursor, w.capacity, k0, k1, 0, 0, nullptr, nullptr);
   NES_HIP_TRY(hipGetLastError());
   return NES_OK;
 }
@@ -663,7 +684,7 @@ int launch_composite(RenderWork &w, bool last, hipStream_t stream) {
                                        w.counts.as<uint32_t>(), w.offsets.as<uint32_t>(),
                                        w.samples.as<uint32_t>(), w.sample_out.as<float4>(),
                                        w.total.as<uint32_t>(), w.capacity, 1, last ? 1 : 0,
-                                       out_ptrs(w));
+                                       out_ptrs(w), nullptr);
   NES_HIP_TRY(hipGetLastError());
   return NES_OK;
 }
@@ -679,7 +700,7 @@ int render_prepare_capacity(RenderWork &w, int precision) {
 
 size_t work_device_bytes(const RenderWork &w) {
   size_t b = 0;
-  for (const DevBuf *d : {&w.views, &w.rays, &w.acc, &w.counts, &w.offsets, &w.ray_sh, &w.samples,
+  for (const DevBuf *d : {&w.views, &w.rays, &w.acc, &w.counts, &w.offsets, &w.ray_sh, &w.tinfo, &w.samples,
                           &w.sample_out, &w.sample_pos, &w.feat, &w.out, &w.total})
     b += d->bytes;
   return b;
@@ -689,7 +710,7 @@ size_t work_device_bytes(const RenderWork &w) {
 // render starts from nothing, and report NES_ENOMEM.
 static int check_budget(RenderWork &w) {
   if (w.budget && work_device_bytes(w) > w.budget) {
-    for (DevBuf *d : {&w.views, &w.rays, &w.acc, &w.counts, &w.offsets, &w.ray_sh, &w.samples,
+    for (DevBuf *d : {&w.views, &w.rays, &w.acc, &w.counts, &w.offsets, &w.ray_sh, &w.tinfo, &w.samples,
                       &w.sample_out, &w.sample_pos, &w.feat, &w.out, &w.total})
       d->release();
     w.capacity = 0;

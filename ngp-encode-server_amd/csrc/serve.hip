@@ -145,7 +145,7 @@ namespace nes {
 
 // Device bytes one batch slot holds for max_rays ray slots, as the render
 // allocates them: per ray slot the ray state (RayData, RayAcc), counts and
-// offsets, the FP16 SH of the ray, per-pixel outputs (RGB24, depth, float RGB, sample count) and the
+// offsets, the FP16 SH of the ray, its single-pass record, per-pixel outputs (RGB24, depth, float RGB, sample count) and the
 // YUV420P frame; per sample (the initial buffer holds 4 per ray slot, at
 // least 2^20) the record, position, MLP output and the 8 feature planes;
 // plus the 1/8 growth slack of every device buffer.  A batch that marches
@@ -154,7 +154,7 @@ static size_t slot_bytes(uint32_t max_rays, int precision) {
   const size_t r = max_rays;
   const size_t cap = std::max<size_t>(r * 4, (size_t)1 << 20);
   const size_t feat = precision == NES_PRECISION_FP32 ? sizeof(float4) : sizeof(uint2);
-  const size_t per_ray = sizeof(RayData) + sizeof(RayAcc) + 8 + 32 + (3 + 4 + 12 + 2) + 2;
+  const size_t per_ray = sizeof(RayData) + sizeof(RayAcc) + 8 + 32 + 8 + (3 + 4 + 12 + 2) + 2;
   const size_t per_sample = 4 + sizeof(float4) + sizeof(float4) + feat * 8;  // 8 XCD part planes
   return (r * per_ray + cap * per_sample) / 8 * 9 + ((size_t)1 << 20);
 }

@@ -65,12 +65,13 @@ def test_poisson_clients_short_run(nes, gpu_ctx):
 
 def slot_bytes(rays, precision_fp32=False):
     """Device bytes of one batch slot: per ray slot RayData 32 + RayAcc 32 +
-    counts/offsets 8 + FP16 SH 32 + outputs 21 + YUV 2; per sample (4 per ray slot, at
+    counts/offsets 8 + FP16 SH 32 + single-pass record 8 + outputs 21 + YUV 2;
+    per sample (4 per ray slot, at
     least 2^20) record 4 + position 16 + MLP output 16 + 8 feature planes
     (8 B FP16, 16 B FP32); 1/8 buffer growth slack; 1 MiB."""
     cap = max(4 * rays, 1 << 20)
     per_sample = 4 + 16 + 16 + (16 if precision_fp32 else 8) * 8
-    return (rays * (32 + 32 + 8 + 32 + 21 + 2) + cap * per_sample) // 8 * 9 + (1 << 20)
+    return (rays * (32 + 32 + 8 + 32 + 8 + 21 + 2) + cap * per_sample) // 8 * 9 + (1 << 20)
 
 
 def test_hbm_budget_decides_batch_slots(nes, gpu_ctx):
