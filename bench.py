@@ -3,7 +3,7 @@
 One step = one render of a stereo pair, 1920x1080 per eye with 64 mm IPD
 (4,147,200 rays), seeded scene (16-level hash grid, 128^3 occupancy blob,
 Xavier MLPs), through the C-ABI (nes_render_device: outputs stay in HBM).
-Prints one JSON line.  --gpus above 1 is refused: this build targets one GPU.
+Prints one JSON line.  --gpus above 1 is refused: the path runs on one GPU.
 """
 import argparse
 import json
@@ -118,7 +118,8 @@ def serving_leg(N, ctx, seconds):
 
 
 def frameprep_leg(N, n_frames=256, w=1280, h=720):
-    """Config 0 frame prep on device: 256 frames per launch, bit-exact kernel."""
+    """Frame prep on device: two 1280x720 RGB24 + depth inputs composited and
+    converted, 256 frames per launch (the bit-exact kernel)."""
     rng = np.random.default_rng(0)
     a = rng.integers(0, 256, (n_frames, h, w, 3), dtype=np.uint8)
     b = rng.integers(0, 256, (n_frames, h, w, 3), dtype=np.uint8)
@@ -166,8 +167,9 @@ def main():
                     help="only run 5 device copies of 1 GiB (memory-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(see DESIGN.md for how the path would shard)", file=sys.stderr)
+        print(f"bench.py: the render path runs on one GPU; --gpus {args.gpus} is not supported "
+              "(requests and tiles shard across GPUs with no exchange; only training would "
+              "all-reduce its gradients)", file=sys.stderr)
         sys.exit(2)
 
     import nes_amd as N

@@ -22,7 +22,7 @@ def assert_planes_equal(got, ref):
 
 
 @pytest.mark.parametrize("mode", [0, 1])
-def test_config0_composite_bit_exact(nes, oracle, mode):
+def test_two_720p_inputs_composite_bit_exact(nes, oracle, mode):
     a, da, b, db = inputs(1280, 720, 0)
     assert_planes_equal(nes.frameprep(a, da, b, db, mode=mode),
                         oracle.frameprep(a, da, b, db, mode=mode))

@@ -50,7 +50,7 @@ def test_full_image_160x120(nes, oracle, gpu_ctx, scene1, prec):
 
 
 @pytest.mark.parametrize("prec", [FP32, MIXED])
-def test_config1_640x480_subset(nes, oracle, gpu_ctx, scene1, prec):
+def test_640x480_fp32_and_mixed_subset(nes, oracle, gpu_ctx, scene1, prec):
     cam, ocam = cams(nes, oracle, 640, 480)
     (out,), st = gpu_ctx.render(cam, precision=prec)
     pix = np.arange(0, 640 * 480, 7, dtype=np.uint32)
@@ -129,7 +129,7 @@ def test_uniform_slab_closed_form_gpu(nes, oracle):
             assert np.abs(o["rgb"][..., 0] - want).max() < 2e-6
 
 
-def test_config2_stereo_1080p_subset(nes, oracle, gpu_ctx, scene1):
+def test_stereo_1080p_subset(nes, oracle, gpu_ctx, scene1):
     """Headline configuration: 1920x1080 per eye, 64 mm IPD; pixels on a
     sparse lattice against the oracle, full frames for determinism."""
     L, oL = cams(nes, oracle, 1920, 1080, eye=-0.032)

@@ -1,6 +1,7 @@
 """Differentiable render on the GPU: forward and gradients against the FP64
 oracle (itself pinned by finite differences), the autograd binding, Adam's
-closed-form first step and a loss that falls on the config-3 setup.
+closed-form first step and a loss that falls when training on 2^18-ray
+batches from 16 poses.
 
 Gradient tolerances: a ReLU whose pre-activation lies closer to zero than the
 FP32 forward's rounding error can switch sides, and the FP32 gradient then
@@ -145,7 +146,7 @@ def test_batch_loss_matches_oracle_and_adam_first_step(nes, oracle, scene1):
         assert np.abs(p1 - p0).max() <= 1e-2 * (1 + 1e-5)
 
 
-def test_config3_training_reduces_loss(nes, oracle, scene1):
+def test_training_2e18_rays_16_poses_reduces_loss(nes, oracle, scene1):
     t, w, o = scene1
     poses, targets = poses_and_targets(nes, oracle, scene1)
     ts, ws, _ = oracle.scene(4, table_init=1, occ=False)
