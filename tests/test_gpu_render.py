@@ -161,3 +161,42 @@ def test_context_budget_bounds_render_buffers(nes, scene1):
         assert e.value.code == -12
         again, _ = ctx.render(small)
         assert np.array_equal(outs[0]["rgb24"], again[0]["rgb24"])
+
+
+@pytest.mark.parametrize("prec", [FP32, MIXED])
+def test_empty_occupancy_renders_background(nes, scene1, prec):
+    """No occupied cell: no samples anywhere, black background, zero depth
+    (the encode and MLP run on an empty sample list)."""
+    t, w, _ = scene1
+    occ = np.zeros(65536, np.uint32)
+    views = [nes.camera(320, 240, nes.orbit_camera(30.0, 20.0, 2.0, e)) for e in (-0.032, 0.032)]
+    with nes.Context(t, w, occ) as ctx:
+        outs, st = ctx.render(views, precision=prec)
+    assert st.samples == 0
+    for o in outs:
+        assert not o["rgb24"].any() and not o["depth"].any() and not o["n_samples"].any()
+
+
+@pytest.mark.parametrize("size", [(1, 1), (3, 5), (7, 9), (65, 33)])
+def test_odd_and_tiny_views_match_oracle(nes, oracle, gpu_ctx, scene1, size):
+    """The render itself accepts any size (only frame prep needs even sizes):
+    partial 8x8 tiles and single pixels give the oracle's sample sets."""
+    w, h = size
+    cam, ocam = cams(nes, oracle, w, h, az=35.0, el=15.0)
+    cam.fov_y_deg = ocam.fov_y_deg = 2.0 if w * h < 16 else 40.0
+    (out,), _ = gpu_ctx.render(cam, precision=FP32)
+    pix = np.arange(w * h, dtype=np.uint32)
+    rgb, depth, ns = oracle_view(oracle, scene1, ocam, pix)
+    assert np.array_equal(out["n_samples"].reshape(-1), ns)
+    assert np.abs(out["rgb"].reshape(-1, 3) - rgb).max() < FP32_RGB_TOL
+
+
+def test_batch_beyond_slot_limit_is_refused(nes, gpu_ctx):
+    """More than 2^25 ray slots in one batch: NES_ERANGE before any launch,
+    and the context keeps working."""
+    big = [nes.camera(8192, 4096, nes.orbit_camera(0.0, 0.0))] * 2
+    with pytest.raises(nes.NesError) as e:
+        gpu_ctx.render_device(big)
+    assert e.value.code == -34
+    (out,), _ = gpu_ctx.render(nes.camera(64, 48, nes.orbit_camera(30.0, 20.0)))
+    assert out["rgb24"].shape[:2] == (48, 64)
