@@ -142,6 +142,37 @@ def frameprep_leg(N, n_frames=256, w=1280, h=720):
             "hbm_frac": bytes_per_frame * n_frames / ms / 1e6 / PEAK_HBM_GBS}
 
 
+def training_leg(N, ctx, occ, steps=12, warmup=3):
+    """Differentiable training step: 2^18 rays per batch drawn from 16 seeded
+    poses at 640x480, targets rendered by the seeded teacher scene, student
+    table U(-1e-4, 1e-4); forward, L2 loss, backward (composite, MLPs on
+    FP32 MFMA, atomic hash scatter) and Adam per step."""
+    rng = np.random.default_rng(3)
+    poses = np.stack([N.orbit_camera(360.0 * rng.random(), 5.0 + 30.0 * rng.random())
+                      for _ in range(16)])
+    outs, _ = ctx.render([N.camera(640, 480, p) for p in poses], precision=N.PRECISION_FP32)
+    targets = np.stack([o["rgb"] for o in outs])
+    ts, ws, _ = N.scene_synthesize(4, table_init=1)
+    rays = 1 << 18
+    with N.Trainer(ts, ws, occ, poses=poses, targets=targets, width=640, height=480,
+                   rays_per_batch=rays, seed=0) as tr:
+        for _ in range(warmup):
+            tr.step()
+        ms, samples, losses = [], [], []
+        for _ in range(steps):
+            loss, st = tr.step()
+            ms.append(st.ms)
+            samples.append(st.samples)
+            losses.append(loss)
+    m = float(np.median(ms))
+    n = float(np.mean(samples))
+    return {"rays_per_step": rays, "ms_per_step": round(m, 3),
+            "mrays_per_s": round(rays / m / 1e3, 2), "samples_per_step": int(n),
+            # forward + data gradient + weight gradient (recompute not credited)
+            "mlp_tflops": round(3 * FLOP_PER_SAMPLE * n / (m * 1e-3) / 1e12, 2),
+            "loss_first": losses[0], "loss_last": losses[-1]}
+
+
 def load_traffic():
     """HBM bytes per launch of encode_parts<f16> on this workload, from the
     committed rocprofv3 PMC summary (FETCH_SIZE + WRITE_SIZE, calibrated
@@ -163,6 +194,9 @@ def main():
     ap.add_argument("--skip-frameprep", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--skip-serve", action="store_true")
+    ap.add_argument("--skip-train", action="store_true")
+    ap.add_argument("--train-only", action="store_true",
+                    help="only the training leg (for profiling its kernels)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 device copies of 1 GiB (memory-counter calibration)")
     args = ap.parse_args()
@@ -185,6 +219,10 @@ def main():
     views = stereo_views(N)
     rays = sum(v.width * v.height for v in views)
     with N.Context(table, weights, occ) as ctx:
+        if args.train_only:
+            print(json.dumps({"training": training_leg(N, ctx, occ, steps=args.steps,
+                                                       warmup=args.warmup)}))
+            return
         wall_m, st_m = time_render(N, ctx, views, N.PRECISION_MIXED, args.steps, args.warmup)
         log(f"mixed: {wall_m / args.steps * 1e3:.3f} ms/step")
         wall_f, st_f = time_render(N, ctx, views, N.PRECISION_FP32, max(3, args.steps // 4),
@@ -229,6 +267,8 @@ def main():
         }
         if not args.skip_frameprep:
             result["frameprep"] = frameprep_leg(N)
+        if not args.skip_train:
+            result["training"] = training_leg(N, ctx, occ)
         if not args.skip_serve:
             srv = serving_leg(N, ctx, args.serve_seconds)
             result["serving"] = srv
