@@ -183,6 +183,7 @@ struct PartLevels {
   bool dense[2];
 };
 
+// Levels of part p (runtime form, used by the training scatter).
 __device__ __forceinline__ PartLevels part_levels(int part) {
   PartLevels p;
 #pragma unroll
@@ -205,85 +206,101 @@ __device__ __forceinline__ PartLevels part_levels(int part) {
   return p;
 }
 
-// Constants of one hash-grid level.
-struct LevelInfo {
-  float scale;
-  uint32_t n1, n1sq, off;
-  bool dense;
+// Compile-time constants of level L for table type Tab (FP16: 4-byte
+// entries, FP32: 8-byte entries; both in the reference layout).
+template <class Tab, int L>
+struct LevelC {
+  static constexpr bool kF16 = std::is_same<Tab, _Float16>::value;
+  static constexpr float kScale = (float)kLevelRes[L];
+  static constexpr uint32_t kN1 = (uint32_t)kLevelRes[L] + 1u;
+  static constexpr bool kDense = L < kDenseLevels;
+  static constexpr uint32_t kOff = kLevelOffset[L];
+  static constexpr int kShift = kF16 ? 2 : 3;  // log2 bytes per entry
 };
 
-__device__ __forceinline__ LevelInfo level_info(int l) {
-  LevelInfo v;
-  int res = 0;
-  uint32_t off = 0;
-#pragma unroll
-  for (int c = 0; c < 16; ++c)
-    if (l == c) {
-      res = kLevelRes[c];
-      off = kLevelOffset[c];
-    }
-  v.scale = (float)res;
-  v.n1 = (uint32_t)res + 1u;
-  v.n1sq = v.n1 * v.n1;
-  v.off = off;
-  v.dense = l < kDenseLevels;
-  return v;
-}
-
-template <class Tab>
-__device__ __forceinline__ float2 encode_level(const Tab *__restrict__ table, const LevelInfo &lv,
-                                               const float xc[3]) {
+// Byte offsets (within level L) of the 8 corners around xc and the
+// trilinear fractions.  Corner c = (x + (c & 1), y + (c >> 1 & 1),
+// z + (c >> 2)); the arithmetic is the reference's index (dense: x + y N1 +
+// z N1^2; hashed: (x ^ y P1 ^ z P2) mod 2^19) carried out directly in byte
+// units: x < 2^19, so ((x ^ h) & M) << s == (x << s) ^ ((h << s) & (M << s)),
+// and (y P1) << s == y (P1 << s) mod 2^32.
+template <class Tab, int L>
+__device__ __forceinline__ void level_corners(const float xc[3], uint32_t boff[8], float fr[3]) {
+  using C = LevelC<Tab, L>;
+  constexpr int S = C::kShift;
   uint32_t i0[3];
-  float fr[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const float pp = xc[a] * lv.scale;
-    const float fl = fminf(floorf(pp), lv.scale - 1.0f);
+    const float pp = xc[a] * C::kScale;
+    const float fl = fminf(floorf(pp), C::kScale - 1.0f);
     i0[a] = (uint32_t)fl;
     fr[a] = pp - fl;
   }
-  typename RawFeat<Tab>::type v[8];
+  const uint32_t x0 = i0[0] << S, x1 = x0 + (1u << S);
+  if constexpr (C::kDense) {
+    constexpr uint32_t dy = C::kN1 << S, dz = (C::kN1 * C::kN1) << S;
+    const uint32_t b = x0 + __umul24(i0[1], dy) + __umul24(i0[2], dz);
 #pragma unroll
-  for (int c = 0; c < 8; c += 2) {
-    uint32_t e[2];
+    for (int c = 0; c < 8; ++c)
+      boff[c] = b + ((c & 1) ? (1u << S) : 0u) + ((c & 2) ? dy : 0u) + ((c & 4) ? dz : 0u);
+  } else {
+    constexpr uint32_t m = ((1u << kLog2T) - 1u) << S;
+    constexpr uint32_t p1 = kPrime1 << S, p2 = kPrime2 << S;
+    const uint32_t hy0 = i0[1] * p1, hy1 = hy0 + p1, hz0 = i0[2] * p2, hz1 = hz0 + p2;
+    const uint32_t h[4] = {(hy0 ^ hz0) & m, (hy1 ^ hz0) & m, (hy0 ^ hz1) & m, (hy1 ^ hz1) & m};
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const uint32_t x = i0[0] + (uint32_t)h, y = i0[1] + ((c >> 1) & 1), z = i0[2] + ((c >> 2) & 1);
-      const uint32_t dense = x + y * lv.n1 + z * lv.n1sq;
-      const uint32_t hash = (x ^ (y * kPrime1) ^ (z * kPrime2)) & ((1u << kLog2T) - 1u);
-      e[h] = lv.off + (lv.dense ? dense : hash);
-    }
-    // one 4-byte (FP16) or 8-byte (FP32) gather per corner: fetching the
-    // (x, x+1) pair as one wider load when it shares an aligned word costs a
-    // second instruction anyway for the lanes that do not pair, and the
-    // wider lanes were measured slower (divergent gathers are bound by the
-    // vector L1's per-lane address rate, not by the request count)
-    v[c] = RawFeat<Tab>::load(table, e[0]);
-    v[c + 1] = RawFeat<Tab>::load(table, e[1]);
+    for (int c = 0; c < 8; ++c) boff[c] = ((c & 1) ? x1 : x0) ^ h[c >> 1];
   }
+}
+
+template <class Tab>
+__device__ __forceinline__ typename RawFeat<Tab>::type load_entry(const char *level_base,
+                                                                  uint32_t boff) {
+  return __ldg(reinterpret_cast<const typename RawFeat<Tab>::type *>(level_base + boff));
+}
+
+__device__ __forceinline__ float2 trilinear(const float fr[3], const float2 fv[8]) {
   const float gx = 1.0f - fr[0], gy = 1.0f - fr[1], gz = 1.0f - fr[2];
   float a0 = 0.0f, a1 = 0.0f;
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
     const float w = ((c & 1) ? fr[0] : gx) * (((c >> 1) & 1) ? fr[1] : gy) *
                     (((c >> 2) & 1) ? fr[2] : gz);
-    const float2 fv = RawFeat<Tab>::unpack(v[c]);
-    a0 = fmaf(w, fv.x, a0);
-    a1 = fmaf(w, fv.y, a1);
+    a0 = fmaf(w, fv[c].x, a0);
+    a1 = fmaf(w, fv[c].y, a1);
   }
   return make_float2(a0, a1);
 }
 
-template <class Tab>
-__device__ __forceinline__ float2 encode_one(const Tab *__restrict__ table, const PartLevels &lp,
-                                             int q, const float xc[3]) {
-  LevelInfo lv;
-  lv.scale = lp.scale[q];
-  lv.n1 = lp.n1[q];
-  lv.n1sq = lp.n1sq[q];
-  lv.off = lp.off[q];
-  lv.dense = lp.dense[q];
-  return encode_level(table, lv, xc);
+// Features of levels LA and LB at one position; all 16 gathers are issued
+// before the first is used.  Each level's base is a wave-uniform pointer, so
+// a gather is one 32-bit VALU offset plus a scalar-based load.
+template <class Tab, int LA, int LB>
+__device__ __forceinline__ void encode_two(const char *base_a, const char *base_b,
+                                           const float xc[3], float2 &fa, float2 &fb) {
+  uint32_t oa[8], ob[8];
+  float fra[3], frb[3];
+  level_corners<Tab, LA>(xc, oa, fra);
+  level_corners<Tab, LB>(xc, ob, frb);
+  typename RawFeat<Tab>::type va[8], vb[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    va[c] = load_entry<Tab>(base_a, oa[c]);
+    vb[c] = load_entry<Tab>(base_b, ob[c]);
+  }
+  float2 fva[8], fvb[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    fva[c] = RawFeat<Tab>::unpack(va[c]);
+    fvb[c] = RawFeat<Tab>::unpack(vb[c]);
+  }
+  fa = trilinear(fra, fva);
+  fb = trilinear(frb, fvb);
+}
+
+template <class Tab, int L>
+__device__ __forceinline__ const char *level_base(const Tab *table) {
+  return reinterpret_cast<const char *>(table) + ((size_t)LevelC<Tab, L>::kOff << LevelC<Tab, L>::kShift);
 }
 
 // Sample positions as the encode reads them.  The FP32 path keeps the exact

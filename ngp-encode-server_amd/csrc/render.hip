@@ -215,6 +215,25 @@ __device__ __forceinline__ float sigmoid_fast(float v) {
 // the corner gathers hit in L2 instead of streaming 64-byte lines from the
 // Infinity Cache.  (In FP32 a hashed level alone is 4 MB.)  Features go to
 // feat[p][sample] as {level p f0, f1, level 15-p f0, f1}.
+template <class Tab, int P>
+__device__ __forceinline__ void encode_part_loop(const typename PosIn<Tab>::type *__restrict__ pos,
+                                                 uint32_t n, uint32_t stride, const Tab *table,
+                                                 typename PartOut<Tab>::type *__restrict__ feat,
+                                                 uint32_t grp, uint32_t blocks_per_part) {
+  constexpr int LA = P, LB = 15 - P;
+  const char *base_a = level_base<Tab, LA>(table), *base_b = level_base<Tab, LB>(table);
+  typename PartOut<Tab>::type *dst = feat + (size_t)P * stride;
+  for (uint32_t s = grp * blockDim.x + threadIdx.x; s < n; s += blocks_per_part * blockDim.x) {
+    float xc[3];
+    PosIn<Tab>::get(pos[s], xc);
+    float2 fa, fb;
+    encode_two<Tab, LA, LB>(base_a, base_b, xc, fa, fb);
+    dst[s] = PartOut<Tab>::pack(fa, fb);
+  }
+}
+
+// Each part is its own instantiation, so level resolutions, offsets and the
+// dense/hashed choice are compile-time constants.
 template <class Tab>
 __global__ __launch_bounds__(256) void encode_parts(const typename PosIn<Tab>::type *__restrict__ pos,
                                                     const uint32_t *__restrict__ n_ptr,
@@ -222,16 +241,16 @@ __global__ __launch_bounds__(256) void encode_parts(const typename PosIn<Tab>::t
                                                     typename PartOut<Tab>::type *__restrict__ feat,
                                                     uint32_t blocks_per_part) {
   const uint32_t n = min(*n_ptr, stride);
-  const int part = (int)(blockIdx.x % kParts);
   const uint32_t grp = blockIdx.x / kParts;
-  const PartLevels lp = part_levels(part);
-  typename PartOut<Tab>::type *dst = feat + (size_t)part * stride;
-  for (uint32_t s = grp * blockDim.x + threadIdx.x; s < n; s += blocks_per_part * blockDim.x) {
-    float xc[3];
-    PosIn<Tab>::get(pos[s], xc);
-    const float2 fa = encode_one(table, lp, 0, xc);
-    const float2 fb = encode_one(table, lp, 1, xc);
-    dst[s] = PartOut<Tab>::pack(fa, fb);
+  switch (blockIdx.x % kParts) {
+    case 0: encode_part_loop<Tab, 0>(pos, n, stride, table, feat, grp, blocks_per_part); break;
+    case 1: encode_part_loop<Tab, 1>(pos, n, stride, table, feat, grp, blocks_per_part); break;
+    case 2: encode_part_loop<Tab, 2>(pos, n, stride, table, feat, grp, blocks_per_part); break;
+    case 3: encode_part_loop<Tab, 3>(pos, n, stride, table, feat, grp, blocks_per_part); break;
+    case 4: encode_part_loop<Tab, 4>(pos, n, stride, table, feat, grp, blocks_per_part); break;
+    case 5: encode_part_loop<Tab, 5>(pos, n, stride, table, feat, grp, blocks_per_part); break;
+    case 6: encode_part_loop<Tab, 6>(pos, n, stride, table, feat, grp, blocks_per_part); break;
+    default: encode_part_loop<Tab, 7>(pos, n, stride, table, feat, grp, blocks_per_part); break;
   }
 }
 
