@@ -293,7 +293,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
 
 // FP16 MFMA MLPs: lane (g, j) of a 16-sample tile loads parts 2g and 2g+1 of
 // sample j -- the layer-1 B operand with K-slot s of group g holding feature
-// part_feature(2g + s/4, s%4) (baked into the W1 fragments).
+// part_feature(2g + s/4, s%4) (baked into the W1 fragments).  Each wave keeps
+// the 20 weight fragments in registers for all its tiles: re-reading them
+// from LDS per tile moves 20 KB of LDS data per 16 samples, and the frame is
+// power-limited, so that energy costs clock in every kernel.
 __global__ __launch_bounds__(256) void mlp_mixed(const uint32_t *__restrict__ samples,
                                                  const uint2 *__restrict__ feat,
                                                  const uint32_t *__restrict__ n_ptr,
@@ -307,17 +310,31 @@ __global__ __launch_bounds__(256) void mlp_mixed(const uint32_t *__restrict__ sa
   __syncthreads();
   const int lane = lane_id(), wave = (int)(threadIdx.x >> 6);
   const int j = lane & 15, g = lane >> 4;
-  const half8 *wf = wl + lane;
+  half8 wf[20];
+#pragma unroll
+  for (int i = 0; i < 20; ++i) wf[i] = wl[64 * i + lane];
   const float4v zero = {0.0f, 0.0f, 0.0f, 0.0f};
   const uint32_t n_tiles = (n + 15) / 16;
-  for (uint32_t tile = blockIdx.x * kEvalWaves + wave; tile < n_tiles;
-       tile += gridDim.x * kEvalWaves) {
+  const uint32_t tile_step = gridDim.x * kEvalWaves;
+  uint32_t tile = blockIdx.x * kEvalWaves + wave;
+  if (tile >= n_tiles) return;
+  // Software pipeline: the sample record and features of the next tile are
+  // loaded while this tile's MFMAs run, its SH (which needs the record) at
+  // the end of the iteration.  The loads are unconditional (indices clamped
+  // to the last sample) so that each wait covers only what it needs.
+  uint32_t sc = min(tile * 16 + (uint32_t)j, n - 1);
+  uint2 fa = feat[(size_t)(2 * g) * stride + sc];
+  uint2 fb = feat[(size_t)(2 * g + 1) * stride + sc];
+  uint2 shw = ray_sh[(size_t)(samples[sc] >> 7) * 4 + g];
+  while (true) {
     const uint32_t s = tile * 16 + (uint32_t)j;
-    const uint32_t sc = min(s, n - 1);  // n >= 1 inside the loop
-    const uint2 fa = feat[(size_t)(2 * g) * stride + sc];
-    const uint2 fb = feat[(size_t)(2 * g + 1) * stride + sc];
-    // SH components 4g .. 4g+3 of the sample's ray (record = slot << 7 | step)
-    const uint2 shw = ray_sh[(size_t)(samples[sc] >> 7) * 4 + g];
+    const uint32_t next = tile + tile_step;
+    const bool more = next < n_tiles;
+    const uint32_t nsc = min(next * 16 + (uint32_t)j, n - 1);
+    const uint32_t nrec = samples[nsc];
+    const uint2 nfa = feat[(size_t)(2 * g) * stride + nsc];
+    const uint2 nfb = feat[(size_t)(2 * g + 1) * stride + nsc];
+    __builtin_amdgcn_sched_barrier(0);  // keep the prefetch ahead of the MFMAs
     const uint32_t words[4] = {fa.x, fa.y, fb.x, fb.y};
     half8 b1;
 #pragma unroll
@@ -328,9 +345,9 @@ __global__ __launch_bounds__(256) void mlp_mixed(const uint32_t *__restrict__ sa
     }
     float4v d1[4];
 #pragma unroll
-    for (int m = 0; m < 4; ++m) d1[m] = mfma(wf[64 * m], b1, zero);
-    float4v d2 = mfma(wf[64 * 4], pack_relu(d1[0], d1[1]), zero);
-    d2 = mfma(wf[64 * 5], pack_relu(d1[2], d1[3]), d2);
+    for (int m = 0; m < 4; ++m) d1[m] = mfma(wf[m], b1, zero);
+    float4v d2 = mfma(wf[4], pack_relu(d1[0], d1[1]), zero);
+    d2 = mfma(wf[5], pack_relu(d1[2], d1[3]), d2);
     const half2v sh01 = __builtin_bit_cast(half2v, shw.x), sh23 = __builtin_bit_cast(half2v, shw.y);
     half8 b3;
     b3[0] = (_Float16)d2[0]; b3[1] = (_Float16)d2[1];
@@ -338,19 +355,26 @@ __global__ __launch_bounds__(256) void mlp_mixed(const uint32_t *__restrict__ sa
     b3[4] = sh01[0]; b3[5] = sh01[1]; b3[6] = sh23[0]; b3[7] = sh23[1];
     float4v d3[4];
 #pragma unroll
-    for (int m = 0; m < 4; ++m) d3[m] = mfma(wf[64 * (6 + m)], b3, zero);
+    for (int m = 0; m < 4; ++m) d3[m] = mfma(wf[6 + m], b3, zero);
     const half8 b4a = pack_relu(d3[0], d3[1]), b4b = pack_relu(d3[2], d3[3]);
     float4v d4[4];
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
-      d4[m] = mfma(wf[64 * (10 + 2 * m)], b4a, zero);
-      d4[m] = mfma(wf[64 * (11 + 2 * m)], b4b, d4[m]);
+      d4[m] = mfma(wf[10 + 2 * m], b4a, zero);
+      d4[m] = mfma(wf[11 + 2 * m], b4b, d4[m]);
     }
-    float4v d5 = mfma(wf[64 * 18], pack_relu(d4[0], d4[1]), zero);
-    d5 = mfma(wf[64 * 19], pack_relu(d4[2], d4[3]), d5);
+    float4v d5 = mfma(wf[18], pack_relu(d4[0], d4[1]), zero);
+    d5 = mfma(wf[19], pack_relu(d4[2], d4[3]), d5);
     if (g == 0 && s < n)
       out[s] = make_float4(__expf(d2[0]), sigmoid_fast(d5[0]), sigmoid_fast(d5[1]),
                            sigmoid_fast(d5[2]));
+    if (!more) break;
+    __builtin_amdgcn_sched_barrier(0);
+    shw = ray_sh[(size_t)(nrec >> 7) * 4 + g];
+    __builtin_amdgcn_sched_barrier(0);  // and the SH load at the end of the iteration
+    tile = next;
+    fa = nfa;
+    fb = nfb;
   }
 }
 
