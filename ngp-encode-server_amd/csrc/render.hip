@@ -303,7 +303,7 @@ __global__ __launch_bounds__(256) void mlp_mixed(const uint32_t *__restrict__ sa
                                                  uint32_t stride,
                                                  const uint2 *__restrict__ ray_sh,
                                                  const half8 *__restrict__ wfrag,
-                                                 float4 *__restrict__ out) {
+                                                 uint2 *__restrict__ out) {
   const uint32_t n = min(*n_ptr, stride);
   __shared__ half8 wl[20 * 64];
   for (int i = (int)threadIdx.x; i < 20 * 64; i += (int)blockDim.x) wl[i] = wfrag[i];
@@ -365,9 +365,15 @@ __global__ __launch_bounds__(256) void mlp_mixed(const uint32_t *__restrict__ sa
     }
     float4v d5 = mfma(wf[18], pack_relu(d4[0], d4[1]), zero);
     d5 = mfma(wf[19], pack_relu(d4[2], d4[3]), d5);
-    if (g == 0 && s < n)
-      out[s] = make_float4(__expf(d2[0]), sigmoid_fast(d5[0]), sigmoid_fast(d5[1]),
-                           sigmoid_fast(d5[2]));
+    if (g == 0 && s < n) {
+      // (log sigma, r, g, b) as FP16; the FP32 composite takes the exp
+      half2v lo, hi;
+      lo.x = (_Float16)d2[0];
+      lo.y = (_Float16)sigmoid_fast(d5[0]);
+      hi.x = (_Float16)sigmoid_fast(d5[1]);
+      hi.y = (_Float16)sigmoid_fast(d5[2]);
+      out[s] = make_uint2(__builtin_bit_cast(uint32_t, lo), __builtin_bit_cast(uint32_t, hi));
+    }
     if (!more) break;
     __builtin_amdgcn_sched_barrier(0);
     shw = ray_sh[(size_t)(nrec >> 7) * 4 + g];
@@ -386,13 +392,22 @@ struct OutPtrs {
   uint16_t *nsamp;
 };
 
+// Per-sample MLP output as the composite reads it: FP32 (sigma, r, g, b), or
+// the mixed path's FP16 (log sigma, r, g, b).
+__device__ __forceinline__ float4 sample_value(const float4 &v) { return v; }
+__device__ __forceinline__ float4 sample_value(const uint2 &v) {
+  const half2v lo = __builtin_bit_cast(half2v, v.x), hi = __builtin_bit_cast(half2v, v.y);
+  return make_float4(__expf((float)lo.x), (float)lo.y, (float)hi.x, (float)hi.y);
+}
+
 // One lane per ray slot, walking the wavefront's rank-major sample range
 // eight ranks at a time (positions from ballots, loads issued together, then
 // composited in order).
+template <class SampleOut>
 __global__ __launch_bounds__(256) void composite(
     uint32_t n_slots, const RayData *__restrict__ rays, RayAcc *acc,
     const uint32_t *__restrict__ counts, const uint32_t *__restrict__ offsets,
-    const uint32_t *__restrict__ samples, const float4 *__restrict__ sout,
+    const uint32_t *__restrict__ samples, const SampleOut *__restrict__ sout,
     const uint32_t *__restrict__ n_ptr, uint32_t capacity, int keep, int last, OutPtrs o,
     const uint2 *__restrict__ tinfo) {
   if (*n_ptr > capacity) return;  // overflowed pass: the host re-renders
@@ -423,7 +438,7 @@ __global__ __launch_bounds__(256) void composite(
   uint32_t before = 0;
   for (uint32_t i0 = 0; i0 < mx; i0 += 8) {  // wavefront-uniform
     if (__ballot(live && !a.done && i0 < cnt) == 0) break;
-    float4 sb[8];
+    SampleOut sb[8];
     uint32_t kb[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
@@ -431,7 +446,7 @@ __global__ __launch_bounds__(256) void composite(
       const uint64_t has = __ballot(cnt > i);
       const uint32_t at = base + before + (uint32_t)__popcll(has & below);
       before += (uint32_t)__popcll(has);
-      sb[u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      sb[u] = SampleOut{};
       kb[u] = 0u;
       if (live && cnt > i && !a.done) {
         sb[u] = sout[at];
@@ -442,7 +457,7 @@ __global__ __launch_bounds__(256) void composite(
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         if (i0 + (uint32_t)u >= cnt || a.done) break;
-        const float4 sm = sb[u];
+        const float4 sm = sample_value(sb[u]);
         const float t = step_t((int)(kb[u] & 127u), ts);
         const float sd = sm.x * kDelta;
         const float alpha = -expm1f(-sd);
@@ -684,14 +699,21 @@ int render_launch_pass(const SceneDev &scene, RenderWork &w, int precision, int 
     mlp_mixed<<<256 * 8, 256, 0, stream>>>(w.samples.as<uint32_t>(), w.feat.as<uint2>(), cursor,
                                            cap, w.ray_sh.as<uint2>(),
                                            reinterpret_cast<const half8 *>(scene.wfrag),
-                                           w.sample_out.as<float4>());
+                                           w.sample_out.as<uint2>());
   }
   NES_HIP_TRY(hipGetLastError());
   if (ev_eval_end) NES_HIP_TRY(hipEventRecord(ev_eval_end, stream));
-  composite<<<blocks, 256, 0, stream>>>(w.n_slots, w.rays.as<RayData>(), w.acc.as<RayAcc>(),
-                                       w.counts.as<uint32_t>(), w.offsets.as<uint32_t>(),
-                                       w.samples.as<uint32_t>(), w.sample_out.as<float4>(),
-                                       cursor, cap, 0, last ? 1 : 0, out_ptrs(w), tinfo);
+  const OutPtrs optr = out_ptrs(w);
+  if (precision == NES_PRECISION_FP32)
+    composite<float4><<<blocks, 256, 0, stream>>>(
+        w.n_slots, w.rays.as<RayData>(), w.acc.as<RayAcc>(), w.counts.as<uint32_t>(),
+        w.offsets.as<uint32_t>(), w.samples.as<uint32_t>(), w.sample_out.as<float4>(), cursor, cap,
+        0, last ? 1 : 0, optr, tinfo);
+  else
+    composite<uint2><<<blocks, 256, 0, stream>>>(
+        w.n_slots, w.rays.as<RayData>(), w.acc.as<RayAcc>(), w.counts.as<uint32_t>(),
+        w.offsets.as<uint32_t>(), w.samples.as<uint32_t>(), w.sample_out.as<uint2>(), cursor, cap,
+        0, last ? 1 : 0, optr, tinfo);
   NES_HIP_TRY(hipGetLastError());
   NES_HIP_TRY(hipMemcpyAsync(w.total_host, cursor, 4, hipMemcpyDeviceToHost, stream));
   return NES_OK;
@@ -723,11 +745,11 @@ int launch_encode_f32(const float *table, RenderWork &w, hipStream_t stream) {
 
 int launch_composite(RenderWork &w, bool last, hipStream_t stream) {
   const unsigned blocks = (unsigned)((w.n_slots + 255) / 256);
-  composite<<<blocks, 256, 0, stream>>>(w.n_slots, w.rays.as<RayData>(), w.acc.as<RayAcc>(),
-                                       w.counts.as<uint32_t>(), w.offsets.as<uint32_t>(),
-                                       w.samples.as<uint32_t>(), w.sample_out.as<float4>(),
-                                       w.total.as<uint32_t>(), w.capacity, 1, last ? 1 : 0,
-                                       out_ptrs(w), nullptr);
+  composite<float4><<<blocks, 256, 0, stream>>>(w.n_slots, w.rays.as<RayData>(), w.acc.as<RayAcc>(),
+                                               w.counts.as<uint32_t>(), w.offsets.as<uint32_t>(),
+                                               w.samples.as<uint32_t>(), w.sample_out.as<float4>(),
+                                               w.total.as<uint32_t>(), w.capacity, 1, last ? 1 : 0,
+                                               out_ptrs(w), nullptr);
   NES_HIP_TRY(hipGetLastError());
   return NES_OK;
 }
