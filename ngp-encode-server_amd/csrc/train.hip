@@ -427,17 +427,32 @@ __global__ __launch_bounds__(256) void train_dcomposite(
 // on hashed levels when x % 8 != 7, so the four adds of one instruction's quad
 // land in one line 7 times in 8: about 4.5 requests per sample and level
 // instead of 16.
+//
+// Level 0 (4,913 dense entries) is hit by every sample on a few hundred
+// lines, and adds to one line queue behind each other: alone it cost twice
+// any other level.  Part 0 therefore sums it in LDS (39 KB) and adds the
+// workgroup's totals once at the end, 16 lanes to a line; the grid is a
+// fixed kScatterBlocks workgroups per part so those flushes stay few.
+constexpr uint32_t kScatterBlocks = 256;
+constexpr uint32_t kLevel0Floats = 2 * (kLevelOffset[1] - kLevelOffset[0]);
+static_assert(kLevelOffset[0] == 0, "level 0 is summed at table entry 0");
+
 __global__ __launch_bounds__(256) void scatter_parts(const float4 *__restrict__ pos,
                                                      const float4 *__restrict__ dfeat,
                                                      const uint32_t *__restrict__ n_ptr,
                                                      uint32_t stride, float *grad_table,
                                                      uint32_t blocks_per_part) {
   constexpr uint32_t kPerBlock = 256 / 4;
+  __shared__ float level0[kLevel0Floats];
   const uint32_t n = min(*n_ptr, stride);
   const int part = (int)(blockIdx.x % kParts);
   const uint32_t grp = blockIdx.x / kParts;
   const PartLevels lp = part_levels(part);
   const uint32_t xb = (threadIdx.x >> 1) & 1u, ft = threadIdx.x & 1u;
+  const bool lds0 = part == 0;  // block-uniform
+  if (lds0)
+    for (uint32_t i = threadIdx.x; i < kLevel0Floats; i += blockDim.x) level0[i] = 0.0f;
+  __syncthreads();
   for (uint32_t s = grp * kPerBlock + (threadIdx.x >> 2); s < n;
        s += blocks_per_part * kPerBlock) {
     const float4 p = pos[s];
@@ -466,10 +481,19 @@ __global__ __launch_bounds__(256) void scatter_parts(const float4 *__restrict__ 
         const uint32_t hash = (x ^ (y * kPrime1) ^ (z * kPrime2)) & ((1u << kLog2T) - 1u);
         const uint32_t e = lp.off[q] + (lp.dense[q] ? dense : hash);
         const float w = wx * ((c & 1) ? fr[1] : 1.0f - fr[1]) * ((c >> 1) ? fr[2] : 1.0f - fr[2]);
-        unsafeAtomicAdd(grad_table + 2 * (size_t)e + ft, w * dv);
+        if (lds0 && q == 0)
+          atomicAdd(&level0[2 * e + ft], w * dv);  // level 0 starts at entry 0
+        else
+          unsafeAtomicAdd(grad_table + 2 * (size_t)e + ft, w * dv);
       }
     }
   }
+  __syncthreads();
+  if (lds0)
+    for (uint32_t i = threadIdx.x; i < kLevel0Floats; i += blockDim.x) {
+      const float v = level0[i];
+      if (v != 0.0f) unsafeAtomicAdd(grad_table + i, v);
+    }
 }
 
 __global__ void adam_kernel(float *p, float *grad, float *m, float *v, size_t n, float lr,
@@ -598,7 +622,7 @@ static int trainer_backward(nes_trainer *t, uint32_t n_rays) {
   ma.grad_w = t->grads + t->n_table;
   mlp_train<true><<<256, 256, 0, s>>>(ma);
   NES_HIP_TRY(hipGetLastError());
-  const uint32_t bpp = part_blocks(w.capacity);
+  const uint32_t bpp = kScatterBlocks;
   scatter_parts<<<bpp * kParts, 256, 0, s>>>(w.sample_pos.as<float4>(), t->dfeat.as<float4>(),
                                              w.total.as<uint32_t>(), w.capacity, t->grads, bpp);
   NES_HIP_TRY(hipGetLastError());
