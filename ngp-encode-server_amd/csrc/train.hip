@@ -428,14 +428,15 @@ __global__ __launch_bounds__(256) void train_dcomposite(
 // land in one line 7 times in 8: about 4.5 requests per sample and level
 // instead of 16.
 //
-// Level 0 (4,913 dense entries) is hit by every sample on a few hundred
-// lines, and adds to one line queue behind each other: alone it cost twice
-// any other level.  Part 0 therefore sums it in LDS (39 KB) and adds the
-// workgroup's totals once at the end, 16 lanes to a line; the grid is a
-// fixed kScatterBlocks workgroups per part so those flushes stay few.
+// Levels 0 and 1 (4,913 and 12,167 dense entries) are hit by every sample on
+// a few hundred lines, and adds to one line queue behind each other: alone
+// they cost 2x and 1.5x any other level.  Parts 0 and 1 therefore sum their
+// dense level in LDS (39 / 97 KB) and add the workgroup's totals once at the
+// end, 16 lanes to a line; the grid is a fixed kScatterBlocks workgroups per
+// part so those flushes stay few.
 constexpr uint32_t kScatterBlocks = 256;
 constexpr uint32_t kLevel0Floats = 2 * (kLevelOffset[1] - kLevelOffset[0]);
-static_assert(kLevelOffset[0] == 0, "level 0 is summed at table entry 0");
+constexpr uint32_t kLevel1Floats = 2 * (kLevelOffset[2] - kLevelOffset[1]);
 
 __global__ __launch_bounds__(256) void scatter_parts(const float4 *__restrict__ pos,
                                                      const float4 *__restrict__ dfeat,
@@ -443,15 +444,17 @@ __global__ __launch_bounds__(256) void scatter_parts(const float4 *__restrict__ 
                                                      uint32_t stride, float *grad_table,
                                                      uint32_t blocks_per_part) {
   constexpr uint32_t kPerBlock = 256 / 4;
-  __shared__ float level0[kLevel0Floats];
+  __shared__ float dense_sum[kLevel1Floats];
   const uint32_t n = min(*n_ptr, stride);
   const int part = (int)(blockIdx.x % kParts);
   const uint32_t grp = blockIdx.x / kParts;
   const PartLevels lp = part_levels(part);
   const uint32_t xb = (threadIdx.x >> 1) & 1u, ft = threadIdx.x & 1u;
-  const bool lds0 = part == 0;  // block-uniform
-  if (lds0)
-    for (uint32_t i = threadIdx.x; i < kLevel0Floats; i += blockDim.x) level0[i] = 0.0f;
+  const bool in_lds = part <= 1;  // block-uniform: level 0 (part 0) or 1 (part 1) in LDS
+  const uint32_t nfl = part == 0 ? kLevel0Floats : kLevel1Floats;
+  const uint32_t off = part == 0 ? 0u : 2u * kLevelOffset[1];
+  if (in_lds)
+    for (uint32_t i = threadIdx.x; i < nfl; i += blockDim.x) dense_sum[i] = 0.0f;
   __syncthreads();
   for (uint32_t s = grp * kPerBlock + (threadIdx.x >> 2); s < n;
        s += blocks_per_part * kPerBlock) {
@@ -481,18 +484,18 @@ __global__ __launch_bounds__(256) void scatter_parts(const float4 *__restrict__ 
         const uint32_t hash = (x ^ (y * kPrime1) ^ (z * kPrime2)) & ((1u << kLog2T) - 1u);
         const uint32_t e = lp.off[q] + (lp.dense[q] ? dense : hash);
         const float w = wx * ((c & 1) ? fr[1] : 1.0f - fr[1]) * ((c >> 1) ? fr[2] : 1.0f - fr[2]);
-        if (lds0 && q == 0)
-          atomicAdd(&level0[2 * e + ft], w * dv);  // level 0 starts at entry 0
+        if (in_lds && q == 0)
+          atomicAdd(&dense_sum[2 * e + ft - off], w * dv);
         else
           unsafeAtomicAdd(grad_table + 2 * (size_t)e + ft, w * dv);
       }
     }
   }
   __syncthreads();
-  if (lds0)
-    for (uint32_t i = threadIdx.x; i < kLevel0Floats; i += blockDim.x) {
-      const float v = level0[i];
-      if (v != 0.0f) unsafeAtomicAdd(grad_table + i, v);
+  if (in_lds)
+    for (uint32_t i = threadIdx.x; i < nfl; i += blockDim.x) {
+      const float v = dense_sum[i];
+      if (v != 0.0f) unsafeAtomicAdd(grad_table + off + i, v);
     }
 }
 
