@@ -15,8 +15,9 @@
 //   mlp_train<bwd>   recomputes the activations of a 16-sample tile (nothing
 //                    per-sample is stored), back-propagates with transposed
 //                    weights on MFMA, and forms the weight gradient as an MFMA
-//                    over samples (tile transposed through LDS), accumulated
-//                    in LDS and flushed once per workgroup
+//                    over samples (tile transposed through LDS), summed over
+//                    the wave's tiles in MFMA accumulators, then reduced in
+//                    LDS and flushed once per workgroup
 //   scatter_parts    XCD-partitioned FP32 atomic adds of dL/dfeature into the
 //                    table gradient (trilinear weights, 8 corners, 2 features)
 //   adam             fused update, zeroes the gradient for the next step
@@ -42,6 +43,15 @@ __device__ __forceinline__ int w3_col(int k) { return k < 16 ? (k == 0 ? -1 : 15
 __device__ __forceinline__ int w1_col(int q) { return part_feature(q >> 2, q & 3); }
 
 constexpr int kTrainWaves = 4;
+
+template <int V>
+struct IC {
+  static constexpr int value = V;
+};
+// Weight-gradient blocks (16 x 16) per layer and their first index in the
+// per-wave accumulator array: layer 5 1x4, layer 4 4x4, layer 3 4x2,
+// layer 2 1x4, layer 1 4x2.
+constexpr int kWgradBlocks = 40;
 
 struct TrainMlpArgs {
   const float4 *pos;
@@ -72,6 +82,10 @@ __global__ __launch_bounds__(256) void mlp_train(TrainMlpArgs a) {
   const int j = lane & 15, g = lane >> 4;
   const float4v zero = {0.0f, 0.0f, 0.0f, 0.0f};
   const uint32_t n_tiles = (n + 15) / 16;
+  // dW blocks summed over all of this wave's tiles in MFMA accumulators
+  float4v wacc[kBwd ? kWgradBlocks : 1];
+#pragma unroll
+  for (int i = 0; i < (kBwd ? kWgradBlocks : 1); ++i) wacc[i] = zero;
   for (uint32_t tile = blockIdx.x * kTrainWaves + wave; tile < n_tiles;
        tile += gridDim.x * kTrainWaves) {
     const uint32_t s = tile * 16 + (uint32_t)j;
@@ -203,26 +217,26 @@ __global__ __launch_bounds__(256) void mlp_train(TrainMlpArgs a) {
       // rows 16bq + 4g + r = part 4bq + g, component r
       if (valid) a.dfeat[(size_t)(4 * bq + g) * a.stride + s] = make_float4(acc[0], acc[1], acc[2], acc[3]);
     }
-    // ---------------- weight gradient: dW_l = sum_s delta_l x_l^T
+    // ---------------- weight gradient: dW_l += sum_s delta_l x_l^T, the
+    // tile's delta and x transposed through LDS, accumulated in wacc
     float(*td)[17] = tr_d[wave];
     float(*tx)[17] = tr_x[wave];
     auto put = [&](float(*t)[17], int b, const float4v &v) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) t[16 * b + 4 * g + r][j] = valid ? v[r] : 0.0f;
     };
-    // out_blocks x in_blocks products, dest(o, k) gives the dW index or -1
-    auto wgrad = [&](int out_blocks, int in_blocks, auto dest) {
+    // OB x IB blocks starting at accumulator BASE
+    auto wgrad = [&](auto ob, auto ib, auto base) {
+      constexpr int OB = decltype(ob)::value, IB = decltype(ib)::value, BASE = decltype(base)::value;
       wave_lds_fence();
-      for (int mo = 0; mo < out_blocks; ++mo)
-        for (int mk = 0; mk < in_blocks; ++mk) {
-          float4v acc = zero;
+#pragma unroll
+      for (int mo = 0; mo < OB; ++mo)
+#pragma unroll
+        for (int mk = 0; mk < IB; ++mk) {
+          float4v acc = wacc[BASE + mo * IB + mk];
 #pragma unroll
           for (int t = 0; t < 4; ++t) acc = mfma32(td[16 * mo + j][4 * t + g], tx[16 * mk + j][4 * t + g], acc);
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int idx = dest(16 * mo + 4 * g + r, 16 * mk + j);
-            if (idx >= 0) atomicAdd(&dW[idx], acc[r]);
-          }
+          wacc[BASE + mo * IB + mk] = acc;
         }
       wave_lds_fence();
     };
@@ -233,7 +247,7 @@ __global__ __launch_bounds__(256) void mlp_train(TrainMlpArgs a) {
       float4v h = {fmaxf(d4[b][0], 0.0f), fmaxf(d4[b][1], 0.0f), fmaxf(d4[b][2], 0.0f), fmaxf(d4[b][3], 0.0f)};
       put(tx, b, h);
     }
-    wgrad(1, 4, [](int o, int k) { return o < 3 ? kW5 + o * 64 + k : -1; });
+    wgrad(IC<1>{}, IC<4>{}, IC<0>{});
     // layer 4: delta e4, x = h3
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
@@ -241,7 +255,7 @@ __global__ __launch_bounds__(256) void mlp_train(TrainMlpArgs a) {
       float4v h = {fmaxf(d3[b][0], 0.0f), fmaxf(d3[b][1], 0.0f), fmaxf(d3[b][2], 0.0f), fmaxf(d3[b][3], 0.0f)};
       put(tx, b, h);
     }
-    wgrad(4, 4, [](int o, int k) { return kW4 + o * 64 + k; });
+    wgrad(IC<4>{}, IC<4>{}, IC<4>{});
     // layer 3: delta e3, x = [out2 | SH]
 #pragma unroll
     for (int b = 0; b < 4; ++b) put(td, b, e3[b]);
@@ -250,10 +264,7 @@ __global__ __launch_bounds__(256) void mlp_train(TrainMlpArgs a) {
       const float4v shv4 = {shq[0], shq[1], shq[2], shq[3]};
       put(tx, 1, shv4);
     }
-    wgrad(4, 2, [](int o, int k) {
-      const int col = w3_col(k);
-      return col < 0 ? -1 : kW3 + o * 31 + col;
-    });
+    wgrad(IC<4>{}, IC<2>{}, IC<20>{});
     // layer 2: delta e2, x = h1
     put(td, 0, e2);
 #pragma unroll
@@ -261,7 +272,7 @@ __global__ __launch_bounds__(256) void mlp_train(TrainMlpArgs a) {
       float4v h = {fmaxf(d1[b][0], 0.0f), fmaxf(d1[b][1], 0.0f), fmaxf(d1[b][2], 0.0f), fmaxf(d1[b][3], 0.0f)};
       put(tx, b, h);
     }
-    wgrad(1, 4, [](int o, int k) { return kW2 + o * 64 + k; });
+    wgrad(IC<1>{}, IC<4>{}, IC<28>{});
     // layer 1: delta e1, x = features
 #pragma unroll
     for (int b = 0; b < 4; ++b) put(td, b, e1[b]);
@@ -270,9 +281,30 @@ __global__ __launch_bounds__(256) void mlp_train(TrainMlpArgs a) {
       const float4v x4 = {xf[b][0], xf[b][1], xf[b][2], xf[b][3]};
       put(tx, b, x4);
     }
-    wgrad(4, 2, [](int o, int k) { return kW1 + o * 32 + w1_col(k); });
+    wgrad(IC<4>{}, IC<2>{}, IC<32>{});
   }
   if (kBwd) {
+    // this wave's sums into the workgroup's LDS dW, dest(o, k) = dW index or -1
+    auto flush = [&](auto ob, auto ib, auto base, auto dest) {
+      constexpr int OB = decltype(ob)::value, IB = decltype(ib)::value, BASE = decltype(base)::value;
+#pragma unroll
+      for (int mo = 0; mo < OB; ++mo)
+#pragma unroll
+        for (int mk = 0; mk < IB; ++mk)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int idx = dest(16 * mo + 4 * g + r, 16 * mk + j);
+            if (idx >= 0) atomicAdd(&dW[idx], wacc[BASE + mo * IB + mk][r]);
+          }
+    };
+    flush(IC<1>{}, IC<4>{}, IC<0>{}, [](int o, int k) { return o < 3 ? kW5 + o * 64 + k : -1; });
+    flush(IC<4>{}, IC<4>{}, IC<4>{}, [](int o, int k) { return kW4 + o * 64 + k; });
+    flush(IC<4>{}, IC<2>{}, IC<20>{}, [](int o, int k) {
+      const int col = w3_col(k);
+      return col < 0 ? -1 : kW3 + o * 31 + col;
+    });
+    flush(IC<1>{}, IC<4>{}, IC<28>{}, [](int o, int k) { return kW2 + o * 64 + k; });
+    flush(IC<4>{}, IC<2>{}, IC<32>{}, [](int o, int k) { return kW1 + o * 32 + w1_col(k); });
     __syncthreads();
     for (int i = (int)threadIdx.x; i < kWeights; i += (int)blockDim.x)
       if (dW[i] != 0.0f) unsafeAtomicAdd(a.grad_w + i, dW[i]);
