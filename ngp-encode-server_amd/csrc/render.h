@@ -100,6 +100,7 @@ inline uint32_t part_blocks(size_t capacity) {
 // in work.acc for the backward).
 int launch_march_emit(const uint32_t *occ, RenderWork &w, int k0, int k1, hipStream_t stream);
 int launch_encode_f32(const float *table, RenderWork &w, hipStream_t stream);
+int launch_mlp_f32(const float *weights, RenderWork &w, hipStream_t stream);
 int launch_composite(RenderWork &w, bool last, hipStream_t stream);
 
 // Enqueue a full render of views on stream; returns after the sample count

@@ -11,13 +11,16 @@
 //   encode_parts  16-level hash-grid encoding, XCD-partitioned: workgroups on
 //                 XCD p encode levels p and 15-p, so each XCD's L2 caches two
 //                 levels of the table
-//   mlp_*         density + colour MLPs over the dense sample list:
-//                   mlp_fp32  FP32 table features, per-lane FP32 VALU MLP
-//                   mlp_mixed FP16 table features, FP16 MFMA MLP, FP32 accum
+//   MLPs          density + colour MLPs over the dense sample list:
+//                   FP32:  mlp_train<false> (train.hip), FP32 features on
+//                          v_mfma_f32_16x16x4_f32
+//                   mixed: mlp_mixed, FP16 features on
+//                          v_mfma_f32_16x16x32_f16, FP32 accumulation
 //   composite     one thread per ray: front-to-back alpha compositing in FP32,
 //                 stops at T < 1e-4, writes RGB24 / depth / float RGB
-// Both precisions share every kernel except the table type in encode_parts
-// and the MLP kernel, so their throughput ratio isolates those.  The sample
+// Both precisions share every kernel except the table type in encode_parts,
+// the MLP kernel and the width of the MLP output, so their throughput ratio
+// isolates those; both MLPs run on matrix cores.  The sample
 // total stays on the device (no host round trip inside a render).
 //
 // Sample geometry (ray, box clip, step positions, occupancy cell) is defined
@@ -156,19 +159,6 @@ __global__ __launch_bounds__(256) void march_emit(
   }
 }
 
-// ------------------------------------------------------------ FP32 eval
-template <int NI, int NO, bool kRelu>
-__device__ __forceinline__ void dense_f32(const float *w, const float *in, float *out) {
-#pragma unroll
-  for (int o = 0; o < NO; ++o) {
-    float acc = 0.0f;
-#pragma unroll
-    for (int i = 0; i < NI; ++i) acc = fmaf(w[o * NI + i], in[i], acc);
-    out[o] = kRelu ? fmaxf(acc, 0.0f) : acc;
-    // keep the scheduler from hoisting every weight read of the layer
-    if ((o & 3) == 3) __asm__ volatile("" ::: "memory");
-  }
-}
 
 constexpr int kEvalWaves = 4;
 
@@ -251,43 +241,6 @@ __global__ __launch_bounds__(256) void encode_parts(const typename PosIn<Tab>::t
     case 5: encode_part_loop<Tab, 5>(pos, n, stride, table, feat, grp, blocks_per_part); break;
     case 6: encode_part_loop<Tab, 6>(pos, n, stride, table, feat, grp, blocks_per_part); break;
     default: encode_part_loop<Tab, 7>(pos, n, stride, table, feat, grp, blocks_per_part); break;
-  }
-}
-
-// FP32 MLPs, one lane per sample, weights in LDS as wave-uniform broadcasts.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void mlp_fp32(
-    const float4 *__restrict__ pos, const float4 *__restrict__ feat,
-    const uint32_t *__restrict__ n_ptr, uint32_t stride, const RayData *__restrict__ rays,
-    const float *__restrict__ Wg, float4 *__restrict__ out) {
-  const uint32_t n = min(*n_ptr, stride);
-  __shared__ __attribute__((aligned(16))) float W[kWeights];
-  for (int i = (int)threadIdx.x; i < kWeights / 4; i += (int)blockDim.x)
-    reinterpret_cast<float4 *>(W)[i] = reinterpret_cast<const float4 *>(Wg)[i];
-  __syncthreads();
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += gridDim.x * blockDim.x) {
-    float f[32];
-#pragma unroll
-    for (int p = 0; p < kParts; ++p) {
-      const float4 v = feat[(size_t)p * stride + i];
-      f[part_feature(p, 0)] = v.x;
-      f[part_feature(p, 1)] = v.y;
-      f[part_feature(p, 2)] = v.z;
-      f[part_feature(p, 3)] = v.w;
-    }
-    const RayData r = load_ray(rays, __float_as_uint(pos[i].w));
-    float h1[64], o2[16], in3[31], h3[64], h4[64], o5[3], sh[16];
-    dense_f32<32, 64, true>(W + kW1, f, h1);
-    dense_f32<64, 16, false>(W + kW2, h1, o2);
-    sh4(r.d, sh);
-#pragma unroll
-    for (int k = 0; k < 16; ++k) in3[k] = sh[k];
-#pragma unroll
-    for (int k = 0; k < 15; ++k) in3[16 + k] = o2[1 + k];
-    dense_f32<31, 64, true>(W + kW3, in3, h3);
-    dense_f32<64, 64, true>(W + kW4, h3, h4);
-    dense_f32<64, 3, false>(W + kW5, h4, o5);
-    out[i] = make_float4(expf(o2[0]), sigmoidf_(o5[0]), sigmoidf_(o5[1]), sigmoidf_(o5[2]));
   }
 }
 
@@ -687,9 +640,7 @@ int render_launch_pass(const SceneDev &scene, RenderWork &w, int precision, int 
                                                            bpp);
     NES_HIP_TRY(hipGetLastError());
     if (ev_encode_end) NES_HIP_TRY(hipEventRecord(ev_encode_end, stream));
-    mlp_fp32<<<256 * 4, 256, 0, stream>>>(w.sample_pos.as<float4>(), w.feat.as<float4>(), cursor,
-                                          cap, w.rays.as<RayData>(), scene.weights_f32,
-                                          w.sample_out.as<float4>());
+    NES_TRY(launch_mlp_f32(scene.weights_f32, w, stream));
   } else {
     encode_parts<_Float16><<<bpp * kParts, 256, 0, stream>>>(w.sample_pos.as<uint2>(), cursor,
                                                               cap, scene.table_f16,

@@ -311,6 +311,26 @@ __global__ __launch_bounds__(256) void mlp_train(TrainMlpArgs a) {
   }
 }
 
+}  // namespace nes
+
+// FP32 forward over the work area's sample list (the FP32 render path and the
+// training forward): density + colour MLPs on v_mfma_f32_16x16x4_f32.
+int nes::launch_mlp_f32(const float *weights, RenderWork &w, hipStream_t stream) {
+  TrainMlpArgs ma{};
+  ma.pos = w.sample_pos.as<float4>();
+  ma.feat = w.feat.as<float4>();
+  ma.n_ptr = w.total.as<uint32_t>();
+  ma.stride = w.capacity;
+  ma.rays = w.rays.as<RayData>();
+  ma.weights = weights;
+  ma.out = w.sample_out.as<float4>();
+  mlp_train<false><<<256 * 2, 256, 0, stream>>>(ma);
+  NES_HIP_TRY(hipGetLastError());
+  return NES_OK;
+}
+
+namespace nes {
+
 // ---------------------------------------------------------------- rays
 struct TrainRayArgs {
   const ViewDesc *poses;
@@ -618,16 +638,7 @@ static int trainer_forward(nes_trainer *t, uint32_t n_rays, const TrainRayArgs &
       continue;
     }
     NES_TRY(launch_encode_f32(t->params, w, s));
-    TrainMlpArgs ma{};
-    ma.pos = w.sample_pos.as<float4>();
-    ma.feat = w.feat.as<float4>();
-    ma.n_ptr = w.total.as<uint32_t>();
-    ma.stride = w.capacity;
-    ma.rays = w.rays.as<RayData>();
-    ma.weights = t->params + t->n_table;
-    ma.out = w.sample_out.as<float4>();
-    mlp_train<false><<<256 * 2, 256, 0, s>>>(ma);
-    NES_HIP_TRY(hipGetLastError());
+    NES_TRY(launch_mlp_f32(t->params + t->n_table, w, s));
     NES_TRY(launch_composite(w, true, s));
     return NES_OK;
   }
