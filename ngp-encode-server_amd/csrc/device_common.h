@@ -144,19 +144,14 @@ __device__ __forceinline__ void sh4(const float d[3], float sh[16]) {
   sh[15] = 0.59004358992664352f * x * (-xx + 3.0f * yy);
 }
 
+// One table entry (2 features) as stored: a float2, or two FP16 in a word.
 template <class Tab> struct RawFeat;
 template <> struct RawFeat<float> {
   typedef float2 type;
-  static __device__ __forceinline__ float2 load(const float *__restrict__ t, uint32_t e) {
-    return __ldg(reinterpret_cast<const float2 *>(t) + e);
-  }
   static __device__ __forceinline__ float2 unpack(float2 v) { return v; }
 };
 template <> struct RawFeat<_Float16> {
   typedef uint32_t type;
-  static __device__ __forceinline__ uint32_t load(const _Float16 *__restrict__ t, uint32_t e) {
-    return __ldg(reinterpret_cast<const uint32_t *>(t) + e);
-  }
   static __device__ __forceinline__ float2 unpack(uint32_t bits) {
     const half2v h = __builtin_bit_cast(half2v, bits);
     return make_float2((float)h.x, (float)h.y);
