@@ -129,13 +129,14 @@ def test_uniform_slab_closed_form_gpu(nes, oracle):
             assert np.abs(o["rgb"][..., 0] - want).max() < 2e-6
 
 
-def test_stereo_1080p_subset(nes, oracle, gpu_ctx, scene1):
+@pytest.mark.parametrize("prec", [FP32, MIXED])
+def test_stereo_1080p_subset(nes, oracle, gpu_ctx, scene1, prec):
     """Headline configuration: 1920x1080 per eye, 64 mm IPD; pixels on a
     sparse lattice against the oracle, full frames for determinism."""
     L, oL = cams(nes, oracle, 1920, 1080, eye=-0.032)
     R, oR = cams(nes, oracle, 1920, 1080, eye=0.032)
-    outs, st = gpu_ctx.render([L, R], precision=MIXED)
-    outs2, st2 = gpu_ctx.render([L, R], precision=MIXED)
+    outs, st = gpu_ctx.render([L, R], precision=prec)
+    outs2, st2 = gpu_ctx.render([L, R], precision=prec)
     assert st.samples == st2.samples and st.rays == 2 * 1920 * 1080
     pix = np.arange(0, 1920 * 1080, 61, dtype=np.uint32)
     for o, o2, oc in zip(outs, outs2, (oL, oR)):
@@ -143,9 +144,9 @@ def test_stereo_1080p_subset(nes, oracle, gpu_ctx, scene1):
         rgb, depth, ns = oracle_view(oracle, scene1, oc, pix)
         assert np.array_equal(o["n_samples"].reshape(-1)[pix], ns)
         p = psnr(o["rgb"].reshape(-1, 3)[pix], rgb)
-        print(f"stereo eye PSNR {p:.1f} dB, max depth err "
+        print(f"precision {prec}: stereo eye PSNR {p:.1f} dB, max depth err "
               f"{np.abs(o['depth'].reshape(-1)[pix] - depth).max():.2e}")
-        assert p >= MIXED_PSNR_DB
+        assert p >= (90.0 if prec == FP32 else MIXED_PSNR_DB)
 
 
 def test_context_budget_bounds_render_buffers(nes, scene1):
