@@ -75,8 +75,10 @@ int nes_orbit_camera(double azimuth_deg, double elevation_deg, double radius,
                      double eye_offset, float c2w[12]);
 
 /* ---------------------------------------------------------------- context */
-#define NES_PRECISION_FP32 0   /* FP32 table, FP32 VALU MLPs */
-#define NES_PRECISION_MIXED 1  /* FP16 table, FP16 MFMA MLPs (FP32 accum) */
+#define NES_PRECISION_FP32 0       /* FP32 table, FP32 MLPs on FP32 MFMA */
+#define NES_PRECISION_MIXED 1      /* FP16 table, FP16 MFMA MLPs (FP32 accum) */
+#define NES_PRECISION_FP32_VALU 2  /* FP32 table, FP32 MLPs per lane on the VALU
+                                      (the plain FP32 GPU path) */
 
 typedef struct nes_ctx nes_ctx;
 
@@ -89,37 +91,58 @@ typedef struct nes_ctx nes_ctx;
 int nes_ctx_create(const nes_scene_desc *scene, uint64_t hbm_budget_bytes,
                    nes_ctx **out);
 int nes_ctx_destroy(nes_ctx *ctx);
-/* Replace the scene parameters (e.g. after training) in place.  Batches a
- * server on this context has in flight may see the old or the new values. */
+/* Replace the scene parameters (e.g. after training).  The new values go to a
+ * second device copy of the scene, which then replaces the current one
+ * between renders: a render or server batch already launched finishes on the
+ * scene it started with, every later one uses the new scene (no frame mixes
+ * the two). */
 int nes_ctx_update_scene(nes_ctx *ctx, const nes_scene_desc *scene);
 
 /* --------------------------------------------------------------- render */
 /* One view's host outputs; any pointer may be NULL.  rgb24 is tightly packed
  * W*H*3, top row first (as RenderedFrame.frame, src/base/video/
  * rendered_frame.cc:14-18); depth is W*H f32 expected depth; rgb_f32 is the
- * unquantised colour W*H*3; n_samples W*H evaluated samples per pixel. */
+ * unquantised colour W*H*3; n_samples W*H evaluated samples per pixel.
+ * depth_gray8 is the W*H GRAY8 depth the reference's RenderedFrame.depth
+ * carries (proto/nes.proto:24, consumed as AV_PIX_FMT_GRAY8 at
+ * src/server.cpp:193-194): g = clamp(floor(fma(d, 255 / depth_far, 0.5)), 0,
+ * 255) in FP32, NaN -> 0, computed on the device; depth_far <= 0 selects
+ * NES_DEPTH_FAR_DEFAULT. */
+#define NES_DEPTH_FAR_DEFAULT 4.0f
 typedef struct nes_view_out {
   uint8_t *rgb24;
   float *depth;
   float *rgb_f32;
   uint16_t *n_samples;
+  uint8_t *depth_gray8;
+  float depth_far;
 } nes_view_out;
 
 typedef struct nes_render_stats {
   uint64_t rays;
   uint64_t samples;         /* evaluated network samples */
   double ms_total;          /* device time of the whole render */
-  double ms_eval;           /* device time of the network kernel */
+  double ms_eval;           /* device time of the network kernels (encode + MLPs) */
   double ms_march;
   double ms_composite;
   double ms_encode;         /* hash-grid encode kernel (part of ms_eval) */
   double ms_mlp;            /* MLP kernel (part of ms_eval) */
+  uint64_t composited;      /* samples composited (those before T < 1e-4) */
+  uint64_t terminated_rays; /* rays that reached T < 1e-4 */
+  uint32_t passes;          /* march passes the render launched */
 } nes_render_stats;
 
 /* Synchronous render of n views (e.g. a stereo pair) in one launch sequence:
  * the in-process replacement of one FrameRequest -> RenderedFrame round trip
- * (src/server.cpp:140-194).  steps_per_pass 0 = all steps in one pass; smaller
- * values march in passes and drop rays once T < 1e-4 (results are identical). */
+ * (src/server.cpp:140-194).
+ * steps_per_pass 0 (default): the whole frame without a host round trip, in
+ *   one pass, or in two when the previous render of this context evaluated
+ *   more than 5 % of its samples behind rays that had already reached
+ *   T < 1e-4: the first pass then gives each ray as many samples as such rays
+ *   needed on average, the second continues the rays still above 1e-4.
+ * steps_per_pass >= 128: exactly one pass.
+ * 1..127: windows of that many march steps, one host round trip each.
+ * Every mode gives bit-identical results. */
 int nes_render(nes_ctx *ctx, int precision, const nes_camera *views, int n_views,
                const nes_view_out *outs, int steps_per_pass,
                nes_render_stats *stats);
@@ -271,9 +294,18 @@ int nes_trainer_set_params(nes_trainer *trainer, const float *table,
  * directions) with the current parameters: rgb is n x 3 ... */
 int nes_trainer_forward(nes_trainer *trainer, const float *ray_o,
                         const float *ray_d, uint32_t n, float *rgb);
-/* ... and the gradient of sum(d_rgb * rgb) for the rays of the last forward. */
-int nes_trainer_backward(nes_trainer *trainer, const float *d_rgb,
+/* ... and the gradient of sum(d_rgb * rgb) (d_rgb n x 3) for those rays with
+ * the current parameters.  The backward recomputes the rays' forward itself,
+ * so it depends on no state of earlier calls. */
+int nes_trainer_backward(nes_trainer *trainer, const float *ray_o,
+                         const float *ray_d, uint32_t n, const float *d_rgb,
                          float *grad_table, float *grad_weights);
+/* Checkpoint: parameters (table and weights), Adam moments and the step
+ * count in one binary file; load restores them into a trainer of the same
+ * scene representation (a mismatched or truncated file changes nothing and
+ * returns NES_EINVAL). */
+int nes_trainer_save(nes_trainer *trainer, const char *path);
+int nes_trainer_load(nes_trainer *trainer, const char *path);
 
 /* --------------------------------------------------------------- serving */
 /* Continuous-batching server: replaces the reference's per-connection loop
@@ -282,8 +314,11 @@ int nes_trainer_backward(nes_trainer *trainer, const float *d_rgb,
  * (src/encode.cpp:42-119).  Clients submit stereo requests from any thread;
  * a scheduler batches waiting requests into free batch slots (one HIP stream
  * and one batch of device buffers each; the HBM budget decides how many
- * slots exist), renders, converts every view to YUV420P on the device and
- * copies the planes to pinned host memory.  As in CameraManager
+ * slots exist and each slot's share of it), renders, converts every view to
+ * YUV420P on the device and copies the planes to host memory.  A batch never
+ * grows past its slot's share: when its samples would not fit, it is
+ * rendered again in passes that give each ray at most capacity / rays
+ * samples each (bit-identical results, more passes).  As in CameraManager
  * (src/base/camera_manager.cc:33-69) only a client's latest pose matters: a
  * newer request supersedes that client's request still waiting for
  * admission (it completes with NES_ECANCELED). */
@@ -300,15 +335,31 @@ typedef struct nes_server_config {
   int sws_mode;              /* NES_SWS_* */
   nes_font *font;            /* text overlay font, or NULL for none */
   uint32_t overlay_flags;    /* NES_OVERLAY_* drawn on every view */
+  /* 1: also emit each view's depth stream, GRAY8 depth (quantised against
+   * depth_far as nes_view_out.depth_gray8) -> YUV420P, the second encoder
+   * input per eye of the reference (src/main.cpp:195-217,
+   * include/base/video/rendered_frame.h:30-31) */
+  uint32_t depth_stream;
+  float depth_far;           /* <= 0: NES_DEPTH_FAR_DEFAULT */
 } nes_server_config;
+
+/* Caller-owned YUV420P planes of one frame: an encoder frame's data[0..2]
+ * and linesize[0..2] (e.g. an AVFrame from av_image_alloc with align 32,
+ * src/base/video/type_managers.cc:116-141).  The server copies the device
+ * planes straight into them (no staging copy); memory from nes_host_alloc is
+ * pinned, so those copies are DMA transfers. */
+typedef struct nes_frame_planes {
+  uint8_t *data[3];
+  int32_t linesize[3];
+} nes_frame_planes;
 
 typedef struct nes_frame_result {
   uint64_t ticket;
   uint64_t index;
   uint32_t n_views;
   uint32_t width[2], height[2];
-  /* tightly packed YUV420P planes (linesizes w and w/2), valid until
-   * nes_server_release */
+  /* YUV420P planes, valid until nes_server_release: the server's tightly
+   * packed planes, or the caller's planes of nes_server_submit_into */
   const uint8_t *y[2];
   const uint8_t *u[2];
   const uint8_t *v[2];
@@ -316,6 +367,15 @@ typedef struct nes_frame_result {
   double queue_ms;   /* submit -> admitted into a batch */
   uint64_t rays, samples;
   int status;        /* NES_OK, NES_ECANCELED (superseded) or an error */
+  int32_t linesize[2][3];      /* of y, u, v per view */
+  /* depth stream planes (depth_stream servers; else NULL) */
+  const uint8_t *depth_y[2];
+  const uint8_t *depth_u[2];
+  const uint8_t *depth_v[2];
+  int32_t depth_linesize[2][3];
+  uint64_t batch;              /* batch the request was rendered in (from 1) */
+  uint32_t batch_requests;     /* requests that batch held */
+  uint32_t batch_passes;       /* march passes of that batch */
 } nes_frame_result;
 
 typedef struct nes_server_stats {
@@ -324,6 +384,12 @@ typedef struct nes_server_stats {
   double p50_ms, p99_ms, mean_ms, max_ms;
   double wall_s;       /* first arrival -> last completion */
   double mrays_per_s;  /* completed rays / wall_s */
+  /* Latency including superseded requests: a superseded request is served
+   * by the frame that replaced it, so it counts with the time from its own
+   * arrival to that frame's completion. */
+  double p50_all_ms, p99_all_ms;
+  uint64_t composited;      /* samples composited (before T < 1e-4) */
+  uint64_t bounded_batches; /* batches rendered in bounded-memory passes */
 } nes_server_stats;
 
 int nes_server_create(nes_ctx *ctx, const nes_server_config *cfg, nes_server **out);
@@ -331,6 +397,18 @@ int nes_server_destroy(nes_server *server);
 /* n_views: 1 or 2 (stereo); even sizes.  NES_EBUSY when the queue is full. */
 int nes_server_submit(nes_server *server, uint32_t client, const nes_camera *views,
                       int n_views, uint64_t *ticket);
+/* Same, with the frame written into caller-owned planes: planes[v] for view
+ * v's scene stream, depth_planes[v] for its depth stream (depth_stream
+ * servers; may be NULL otherwise).  Each plane must hold its rows at the
+ * given linesize (>= w for Y, >= w/2 for U and V) and stay valid until the
+ * request completes. */
+int nes_server_submit_into(nes_server *server, uint32_t client, const nes_camera *views,
+                           int n_views, const nes_frame_planes *planes,
+                           const nes_frame_planes *depth_planes, uint64_t *ticket);
+/* hold != 0 stops admission (waiting requests stay queued, batches in flight
+ * finish); hold == 0 resumes it.  Lets a caller fill the queue first so the
+ * next batch takes every request that fits. */
+int nes_server_hold(nes_server *server, int hold);
 /* timeout_ms < 0 waits forever; NES_EAGAIN on timeout. */
 int nes_server_wait(nes_server *server, uint64_t ticket, int timeout_ms,
                     nes_frame_result *out);
@@ -389,6 +467,9 @@ int nes_sws_vchroma_filter(int h, int16_t *coef, int32_t *pos, int max_taps,
  * GPU runtime. */
 int nes_device_alloc(size_t bytes, void **ptr);
 int nes_device_free(void *ptr);
+/* Pinned host memory (e.g. for nes_frame_planes the server writes by DMA). */
+int nes_host_alloc(size_t bytes, void **ptr);
+int nes_host_free(void *ptr);
 int nes_memcpy_h2d(void *dst, const void *src, size_t bytes);
 int nes_memcpy_d2h(void *dst, const void *src, size_t bytes);
 /* Device-to-device copy (a plain copy of known size, used to calibrate

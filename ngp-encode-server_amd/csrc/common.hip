@@ -111,6 +111,19 @@ int nes_device_free(void *ptr) {
   return NES_OK;
 }
 
+int nes_host_alloc(size_t bytes, void **ptr) {
+  if (!ptr) return NES_EINVAL;
+  NES_TRY(nes::ensure_device());
+  NES_HIP_TRY(hipHostMalloc(ptr, bytes ? bytes : 1));
+  return NES_OK;
+}
+
+int nes_host_free(void *ptr) {
+  NES_TRY(nes::ensure_device());
+  if (ptr) NES_HIP_TRY(hipHostFree(ptr));
+  return NES_OK;
+}
+
 int nes_memcpy_h2d(void *dst, const void *src, size_t bytes) {
   NES_TRY(nes::ensure_device());
   NES_HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));

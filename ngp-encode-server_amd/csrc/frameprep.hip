@@ -20,6 +20,7 @@
 // halo) once from HBM, composites A/B, writes luma and parks the 15-bit
 // chroma intermediates in LDS; phase 2 runs the vertical chroma filter out of
 // LDS.  Inputs are read once (+~9% halo), outputs written once.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -162,6 +163,8 @@ std::mutex g_filter_mu;
 std::map<int, DevVFilter> g_filters;
 }  // namespace
 
+static bool filter_fits_tiles(const VFilter &f, int h);
+
 int get_dev_vfilter(int h, DevVFilter *out) {
   std::lock_guard<std::mutex> lk(g_filter_mu);
   auto it = g_filters.find(h);
@@ -170,7 +173,7 @@ int get_dev_vfilter(int h, DevVFilter *out) {
     return NES_OK;
   }
   VFilter f = build_vchroma_filter(h);
-  if (f.taps < 1 || f.taps > 16) {
+  if (f.taps < 1 || f.taps > 16 || !filter_fits_tiles(f, h)) {
     set_last_error("unsupported vertical filter size");
     return NES_ERANGE;
   }
@@ -192,6 +195,19 @@ constexpr int kFpQuads = 64;               // 4-pixel quads per tile row
 constexpr int kFpCRows = 32;               // chroma rows per tile
 constexpr int kFpRows = 2 * kFpCRows + 16; // LDS source rows (halo + slack)
 constexpr int kFpThreads = 256;
+
+// Source rows one tile of kFpCRows chroma rows reads (the kernel's lo..hi):
+// every tile of a filter must fit the kFpRows rows of the LDS tile.
+static bool filter_fits_tiles(const VFilter &f, int h) {
+  const int ch = h >> 1;
+  for (int k0 = 0; k0 < ch; k0 += kFpCRows) {
+    const int k1 = std::min(k0 + kFpCRows, ch);
+    const int lo = std::min(f.pos[k0], 2 * k0);
+    const int hi = std::min(std::max(f.pos[k1 - 1] + f.taps - 1, 2 * k1 - 1), h - 1);
+    if (lo < 0 || hi - lo + 1 > kFpRows) return false;
+  }
+  return true;
+}
 
 struct FrameprepArgs {
   const uint8_t *rgb_a;
@@ -393,14 +409,26 @@ int launch_frameprep(const uint8_t *rgb_a, const float *depth_a,
 // GRAY8 full range -> YUV420P limited range (the depth stream).  Luma:
 // hScale8To15 (g << 7) then lumRangeFromJpeg_c; chroma planes of a gray
 // source hold the constant 1 << 14 before the vertical filter.
+// kDepth: the source is FP32 expected depth, quantised to the GRAY8 depth
+// plane on the fly (depth_gray8, `scale` = 255 / far); that plane is also
+// written to `gray_out` when it is not null.
+template <bool kDepth>
 __global__ void gray_kernel(const uint8_t *g, int g_stride, int w, int h,
                             uint8_t *y, int y_stride, uint8_t *u, int u_stride,
                             uint8_t *v, int v_stride, const int16_t *coef,
-                            const int32_t *pos, int taps, int mode) {
+                            const int32_t *pos, int taps, int mode, const float *depth,
+                            float scale, uint8_t *gray_out) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int r = blockIdx.y;
   if (x < w) {
-    int v15 = (int)g[(size_t)r * g_stride + x] << 7;
+    int gv;
+    if (kDepth) {
+      gv = depth_gray8(depth[(size_t)r * w + x], scale);
+      if (gray_out) gray_out[(size_t)r * w + x] = (uint8_t)gv;
+    } else {
+      gv = g[(size_t)r * g_stride + x];
+    }
+    int v15 = gv << 7;
     v15 = (v15 * 14071 + 33561947) >> 14;
     y[(size_t)r * y_stride + x] = (uint8_t)min(max((v15 + 64) >> 7, 0), 255);
   }
@@ -422,6 +450,22 @@ __global__ void gray_kernel(const uint8_t *g, int g_stride, int w, int h,
     v[(size_t)k * v_stride + x] = (uint8_t)out;
   }
   (void)pos;
+}
+
+int launch_depth_yuv(const float *depth, int w, int h, float scale, uint8_t *gray, uint8_t *y,
+                     int y_stride, uint8_t *u, int u_stride, uint8_t *v, int v_stride,
+                     int sws_mode, hipStream_t stream) {
+  if (!depth || !y || !u || !v || w < 2 || h < 2 || (w & 1) || (h & 1)) return NES_EINVAL;
+  if (y_stride < w || u_stride < w / 2 || v_stride < w / 2) return NES_EINVAL;
+  if (sws_mode != NES_SWS_X86 && sws_mode != NES_SWS_C) return NES_EINVAL;
+  DevVFilter flt;
+  NES_TRY(get_dev_vfilter(h, &flt));
+  dim3 grid((unsigned)((w + 255) / 256), (unsigned)h);
+  gray_kernel<true><<<grid, 256, 0, stream>>>(nullptr, 0, w, h, y, y_stride, u, u_stride, v,
+                                              v_stride, flt.coef, flt.pos, flt.taps, sws_mode,
+                                              depth, scale, gray);
+  NES_HIP_TRY(hipGetLastError());
+  return NES_OK;
 }
 
 namespace {
@@ -528,8 +572,9 @@ int nes_gray8_to_yuv420p(const uint8_t *gray, int w, int h, int g_stride,
   uint8_t *d_y = g_out.as<uint8_t>(), *d_u = d_y + px, *d_v = d_u + cw * chh;
   NES_HIP_TRY(hipMemcpy2DAsync(d_g, w, gray, g_stride, w, h, hipMemcpyHostToDevice, s));
   dim3 grid((unsigned)((w + 255) / 256), (unsigned)h);
-  gray_kernel<<<grid, 256, 0, s>>>(d_g, w, w, h, d_y, w, d_u, (int)cw, d_v, (int)cw,
-                                   flt.coef, flt.pos, flt.taps, sws_mode);
+  gray_kernel<false><<<grid, 256, 0, s>>>(d_g, w, w, h, d_y, w, d_u, (int)cw, d_v, (int)cw,
+                                          flt.coef, flt.pos, flt.taps, sws_mode, nullptr, 0.0f,
+                                          nullptr);
   NES_HIP_TRY(hipGetLastError());
   NES_HIP_TRY(hipMemcpy2DAsync(y, y_stride, d_y, w, w, h, hipMemcpyDeviceToHost, s));
   NES_HIP_TRY(hipMemcpy2DAsync(u, u_stride, d_u, cw, cw, chh, hipMemcpyDeviceToHost, s));

@@ -72,6 +72,17 @@ inline double rand_uniform(uint64_t seed, uint64_t stream, uint64_t i) {
   return (double)(rand_u64(seed, stream, i) >> 11) * 0x1.0p-53;
 }
 
+// ------------------------------------------------------------ GRAY8 depth
+// The depth plane of a RenderedFrame (proto/nes.proto:24) is GRAY8:
+// g = clamp(floor(fma(d, 255 / far, 0.5)), 0, 255) in FP32 (NaN -> 0).
+inline float depth_scale(float far) {
+  return 255.0f / (far > 0.0f ? far : NES_DEPTH_FAR_DEFAULT);
+}
+__host__ __device__ inline uint8_t depth_gray8(float d, float scale) {
+  const float q = floorf(fmaf(d, scale, 0.5f));
+  return (uint8_t)fminf(fmaxf(q, 0.0f), 255.0f);
+}
+
 // ------------------------------------------------------------ frame prep
 struct VFilter {
   int taps = 0;                 // filter size
@@ -99,6 +110,13 @@ int launch_frameprep(const uint8_t *rgb_a, const float *depth_a,
                      size_t y_frame_bytes, uint8_t *u, int u_stride,
                      size_t u_frame_bytes, uint8_t *v, int v_stride,
                      size_t v_frame_bytes, int sws_mode, hipStream_t stream);
+
+// The depth stream of one view straight from FP32 depth: quantise to GRAY8
+// (scale = 255 / far; the plane goes to `gray` when not null) and convert
+// GRAY8 -> YUV420P as nes_gray8_to_yuv420p does.  `depth` is w*h, tight.
+int launch_depth_yuv(const float *depth, int w, int h, float scale, uint8_t *gray, uint8_t *y,
+                     int y_stride, uint8_t *u, int u_stride, uint8_t *v, int v_stride,
+                     int sws_mode, hipStream_t stream);
 
 // Grow-only device scratch buffer.
 struct DevBuf {

@@ -1,6 +1,8 @@
 // render.h -- device data structures of the render pipeline and the context.
 #pragma once
 
+#include <memory>
+
 #include "nes_internal.h"
 
 namespace nes {
@@ -33,18 +35,43 @@ struct RayAcc {  // 32 bytes, compositing state carried across passes
 // Sample record: ray slot in the high 25 bits, step index in the low 7.
 constexpr uint32_t kMaxSlots = 1u << 25;
 
+// Per ray slot, the march writes counts[slot] = samples emitted by the pass in
+// the low 16 bits and the step the next pass resumes at in the high 16 bits.
+constexpr uint32_t kCountMask = 0xffffu;
+
+// Words of RenderWork::total (device) and RenderWork::total_host (pinned).
+//   device: [0] sample cursor of the current pass, [1] samples composited,
+//           [2] rays with at least one composited sample, [3] rays that
+//           reached T < 1e-4, [4] samples composited by those rays,
+//           [5] samples evaluated by all passes of the frame
+//   host:   [0] first-pass sample total, [1] later-pass sample total (0 for
+//           a single pass), [2..6] device words [1..5] of the frame
+constexpr int kTotCursor = 0, kTotComposited = 1, kTotHitRays = 2, kTotTerminated = 3,
+              kTotTermSamples = 4, kTotEvaluated = 5, kTotWords = 6;
+
 struct RenderWork {
   DevBuf views, rays, acc, counts, offsets, samples, sample_out, sample_pos, feat, out, total;
   DevBuf ray_sh;                  // mixed path: per ray slot 16 FP16 SH values
   DevBuf tinfo;                   // single-pass renders: per ray slot t_start, pixel
+  DevBuf gray;                    // GRAY8 depth staging of nes_render_fetch
   uint32_t *total_host = nullptr;  // pinned
-  hipEvent_t ev[8] = {};
+  hipEvent_t ev[12] = {};
   uint32_t n_views = 0, n_slots = 0;
   uint32_t capacity = 0;          // samples the sample buffers hold
   int capacity_precision = -1;    // feature layout the buffers were sized for
   size_t out_px = 0;
   std::vector<ViewDesc> host_views;
   uint64_t budget = 0;            // device bytes the buffers may hold, 0 = no limit
+  // Early termination learned from the previous frame of this work area:
+  // samples the first pass gives each ray (kMaxSteps = one single pass).
+  uint32_t first_pass_cap = kMaxSteps;
+  uint32_t frame_passes = 1;      // passes the last frame launched
+};
+
+// What one frame evaluated and composited (for early-termination stats).
+struct FrameCounts {
+  uint64_t evaluated = 0, composited = 0, hit_rays = 0, terminated = 0;
+  uint32_t passes = 1;
 };
 
 // Device bytes currently held by a RenderWork's buffers.
@@ -60,6 +87,8 @@ struct SceneDev {
 
 int scene_upload(SceneDev &s, const nes_scene_desc *desc);
 void scene_free(SceneDev &s);
+// An empty scene copy whose device memory is freed with the last reference.
+std::shared_ptr<SceneDev> new_scene();
 // Permute the MLP weights into the A-operand fragments of the 20
 // v_mfma_f32_16x16x32_f16 instructions one 16-sample tile uses.
 void build_mlp_fragments(const float *w, _Float16 *frag);
@@ -81,9 +110,39 @@ OutLayout out_layout(size_t px);
 int render_setup(RenderWork &w, const nes_camera *views, int n_views, hipStream_t stream);
 int render_prepare_capacity(RenderWork &w, int precision);
 int render_reserve_samples(RenderWork &w, int precision, size_t capacity);
+// One pass: march steps [k0, k1) (or, with resume, from where the ray's
+// previous pass stopped), at most `cap` samples per ray, then encode, MLPs
+// and composite.
 int render_launch_pass(const SceneDev &scene, RenderWork &w, int precision, int k0, int k1,
                        bool first, bool last, hipStream_t stream, hipEvent_t ev_eval_begin,
-                       hipEvent_t ev_eval_end, hipEvent_t ev_encode_end);
+                       hipEvent_t ev_eval_end, hipEvent_t ev_encode_end, uint32_t cap = kMaxSteps,
+                       bool resume = false);
+// A whole frame without a host round trip: one pass when the work area's
+// previous frame composited (almost) every sample it evaluated, else two:
+// the first gives each ray at most first_pass_cap samples, the second
+// continues the rays still above T = 1e-4 with all their remaining samples.
+// The pass totals and composite counters land in total_host (read them after
+// the stream has passed the launches).  Results are bit-identical to a single
+// pass.
+// ev (optional): 8 events recorded per pass p at [4p]: pass start, [4p+1]
+// encode start, [4p+2] encode end, [4p+3] MLP end.
+int render_launch_frame(const SceneDev &scene, RenderWork &w, int precision, hipStream_t stream,
+                        const hipEvent_t *ev = nullptr);
+// After the frame's stream work is complete: the largest pass total (a value
+// above w.capacity means the frame must be launched again after growth) and
+// the counts; updates first_pass_cap for the next frame.
+uint32_t render_frame_max_total(const RenderWork &w);
+FrameCounts render_frame_finish(RenderWork &w);
+// Device bytes the sample buffers of `capacity` samples take (all buffers
+// that grow with the sample count).
+size_t sample_buffer_bytes(size_t capacity, int precision);
+// Whether growing the sample buffers to `capacity` keeps the work area
+// within its budget.
+bool render_growth_fits(const RenderWork &w, int precision, size_t capacity);
+// A frame within the current sample buffers, whatever the scene: passes that
+// give each ray at most capacity / rays samples, each resuming where the
+// previous one stopped (bit-identical to one pass; no overflow possible).
+int render_launch_bounded(const SceneDev &scene, RenderWork &w, int precision, hipStream_t stream);
 
 // Workgroups per XCD part of the encode (and of the training scatter) for a
 // sample buffer of `capacity`: about one sample per thread, since the
@@ -133,9 +192,20 @@ std::string overlay_now_text();
 }  // namespace nes
 
 struct nes_ctx {
-  nes::SceneDev scene;
+  // The scene renders read.  nes_ctx_update_scene uploads into another copy
+  // and swaps the pointer between renders; a render or server batch holds a
+  // reference to the copy it started with until its device work is complete,
+  // so no frame mixes two scenes.  `spare` is the previous copy, reused by
+  // the next update once nothing holds it.
+  std::shared_ptr<nes::SceneDev> scene, spare;
+  std::mutex scene_mu;  // guards scene / spare
   nes::RenderWork work;
   hipStream_t stream = nullptr;
   uint64_t budget = 0;
-  std::mutex mu;
+  std::mutex mu;  // serialises the context's own renders (work, stream)
 };
+
+namespace nes {
+// The context's current scene (a reference that keeps it alive).
+std::shared_ptr<SceneDev> ctx_scene(nes_ctx *ctx);
+}  // namespace nes

@@ -41,21 +41,44 @@ namespace nes {
 // sample total never leaves the device: later kernels read it from the
 // cursor, and writes beyond `capacity` are dropped (the host re-renders with a
 // larger buffer when the final total exceeds it).  counts[slot] receives the
-// ray's sample count, offsets[slot] the wavefront's range start.
-__global__ __launch_bounds__(256) void march_emit(
-    const ViewDesc *__restrict__ views, uint32_t n_views, uint32_t n_slots,
-    const uint32_t *__restrict__ occ, RayData *rays, RayAcc *acc, uint32_t *counts,
-    uint32_t *offsets, uint32_t *samples, float4 *__restrict__ pos, uint32_t *cursor,
-    uint32_t capacity, int k_begin, int k_end, int first, int packed, uint2 *ray_sh,
-    uint2 *tinfo) {
+// ray's sample count (and the step a later pass resumes at, see kCountMask),
+// offsets[slot] the wavefront's range start.  A ray emits at most `cap`
+// samples per pass; with `resume` it starts where its previous pass stopped.
+struct MarchArgs {
+  const ViewDesc *views;
+  uint32_t n_views, n_slots;
+  const uint32_t *occ;
+  RayData *rays;
+  RayAcc *acc;
+  uint32_t *counts, *offsets, *samples;
+  float4 *pos;
+  uint32_t *cursor;  // the frame counters (RenderWork::total)
+  uint32_t capacity, cap;
+  int k_begin, k_end, first, packed, resume;
+  uint2 *ray_sh;
+  uint2 *tinfo;
+};
+
+__global__ __launch_bounds__(256) void march_emit(MarchArgs m) {
+  const ViewDesc *__restrict__ views = m.views;
+  const uint32_t n_views = m.n_views, n_slots = m.n_slots;
+  const uint32_t *__restrict__ occ = m.occ;
+  RayData *rays = m.rays;
+  RayAcc *acc = m.acc;
+  uint32_t *counts = m.counts, *offsets = m.offsets, *samples = m.samples;
+  float4 *__restrict__ pos = m.pos;
+  const uint32_t capacity = m.capacity;
+  const int packed = m.packed;
+  uint2 *ray_sh = m.ray_sh, *tinfo = m.tinfo;
   const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = lane_id();
   uint32_t cnt = 0;
+  uint32_t knext = kMaxSteps;           // where a later pass resumes this ray
   uint32_t bits[4] = {0u, 0u, 0u, 0u};  // occupied steps of this pass
   RayData r;
   bool active = false;
   if (slot < n_slots) {
-    if (first) {
+    if (m.first) {
       uint32_t vi = 0;
       for (uint32_t i = 1; i < n_views; ++i)
         if (slot >= views[i].slot_base) vi = i;
@@ -92,13 +115,19 @@ __global__ __launch_bounds__(256) void march_emit(
       active = !acc[slot].done;
     }
     if (active) {
-      const int kend = min(k_end, (int)r.n_steps);
-      for (int k = k_begin; k < kend; ++k) {
+      int k = m.k_begin;
+      if (m.resume) k = max(k, (int)(counts[slot] >> 16));
+      const int kend = min(m.k_end, (int)r.n_steps);
+      knext = (uint32_t)max(kend, k);
+      for (; k < kend; ++k) {
         float x[3], t;
         step_pos(r, k, x, &t);
         if (occupied(occ, x)) {
           bits[k >> 5] |= 1u << (k & 31);
-          ++cnt;
+          if (++cnt == m.cap) {
+            knext = (uint32_t)k + 1u;
+            break;
+          }
         }
       }
     }
@@ -122,10 +151,13 @@ __global__ __launch_bounds__(256) void march_emit(
   }
   const uint32_t wave_total = wave_sum(cnt);
   uint32_t base = 0;
-  if (lane == 0 && wave_total) base = atomicAdd(cursor, wave_total);
+  if (lane == 0 && wave_total) {
+    base = atomicAdd(m.cursor + kTotCursor, wave_total);
+    atomicAdd(m.cursor + kTotEvaluated, wave_total);
+  }
   base = __shfl(base, 0);
   if (slot < n_slots) {
-    counts[slot] = cnt;
+    counts[slot] = cnt | (knext << 16);
     offsets[slot] = base;
   }
   const uint32_t mx = wave_max(cnt);
@@ -161,6 +193,57 @@ __global__ __launch_bounds__(256) void march_emit(
 
 
 constexpr int kEvalWaves = 4;
+
+// ------------------------------------------------------ FP32 VALU eval
+// The plain FP32 path: FP32 table features, one lane per sample, every MAC a
+// VALU FMA with the weights in LDS (wave-uniform broadcasts).
+template <int NI, int NO, bool kRelu>
+__device__ __forceinline__ void dense_f32(const float *w, const float *in, float *out) {
+#pragma unroll
+  for (int o = 0; o < NO; ++o) {
+    float acc = 0.0f;
+#pragma unroll
+    for (int i = 0; i < NI; ++i) acc = fmaf(w[o * NI + i], in[i], acc);
+    out[o] = kRelu ? fmaxf(acc, 0.0f) : acc;
+    // keep the scheduler from hoisting every weight read of the layer
+    if ((o & 3) == 3) __asm__ volatile("" ::: "memory");
+  }
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void mlp_fp32_valu(
+    const float4 *__restrict__ pos, const float4 *__restrict__ feat,
+    const uint32_t *__restrict__ n_ptr, uint32_t stride, const RayData *__restrict__ rays,
+    const float *__restrict__ Wg, float4 *__restrict__ out) {
+  const uint32_t n = min(*n_ptr, stride);
+  __shared__ __attribute__((aligned(16))) float W[kWeights];
+  for (int i = (int)threadIdx.x; i < kWeights / 4; i += (int)blockDim.x)
+    reinterpret_cast<float4 *>(W)[i] = reinterpret_cast<const float4 *>(Wg)[i];
+  __syncthreads();
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    float f[32];
+#pragma unroll
+    for (int p = 0; p < kParts; ++p) {
+      const float4 v = feat[(size_t)p * stride + i];
+      f[part_feature(p, 0)] = v.x;
+      f[part_feature(p, 1)] = v.y;
+      f[part_feature(p, 2)] = v.z;
+      f[part_feature(p, 3)] = v.w;
+    }
+    const RayData r = load_ray(rays, __float_as_uint(pos[i].w));
+    float h1[64], o2[16], in3[31], h3[64], h4[64], o5[3], sh[16];
+    dense_f32<32, 64, true>(W + kW1, f, h1);
+    dense_f32<64, 16, false>(W + kW2, h1, o2);
+    sh4(r.d, sh);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) in3[k] = sh[k];
+#pragma unroll
+    for (int k = 0; k < 15; ++k) in3[16 + k] = o2[1 + k];
+    dense_f32<31, 64, true>(W + kW3, in3, h3);
+    dense_f32<64, 64, true>(W + kW4, h3, h4);
+    dense_f32<64, 3, false>(W + kW5, h4, o5);
+    out[i] = make_float4(expf(o2[0]), sigmoidf_(o5[0]), sigmoidf_(o5[1]), sigmoidf_(o5[2]));
+  }
+}
 
 // ----------------------------------------------------------- mixed eval
 // MFMA tile: v_mfma_f32_16x16x32_f16, D[16x16] += A[16x32] B[32x16].
@@ -362,7 +445,7 @@ __global__ __launch_bounds__(256) void composite(
     const uint32_t *__restrict__ counts, const uint32_t *__restrict__ offsets,
     const uint32_t *__restrict__ samples, const SampleOut *__restrict__ sout,
     const uint32_t *__restrict__ n_ptr, uint32_t capacity, int keep, int last, OutPtrs o,
-    const uint2 *__restrict__ tinfo) {
+    const uint2 *__restrict__ tinfo, uint32_t *stats) {
   if (*n_ptr > capacity) return;  // overflowed pass: the host re-renders
   const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = lane_id();
@@ -380,7 +463,7 @@ __global__ __launch_bounds__(256) void composite(
     } else {
       a = acc[slot];
     }
-    cnt = counts[slot];
+    cnt = counts[slot] & kCountMask;
     base = offsets[slot];
   }
   base = __shfl(base, 0);  // lane 0 is valid whenever any lane is
@@ -425,6 +508,24 @@ __global__ __launch_bounds__(256) void composite(
       }
     }
   }
+  if (last && stats) {
+    // frame counters: samples composited, rays that composited any, rays
+    // that reached T < 1e-4 and the samples those composited
+    const bool px = valid && a.pad != 0xffffffffu;
+    const bool term = px && a.n_samples > 0 && a.T < kTStop;
+    const uint32_t nc = wave_sum(px ? a.n_samples : 0u);
+    const uint32_t nt = wave_sum(term ? a.n_samples : 0u);
+    const uint32_t nh = (uint32_t)__popcll(__ballot(px && a.n_samples > 0));
+    const uint32_t nr = (uint32_t)__popcll(__ballot(term));
+    if (lane == 0 && nc) {
+      atomicAdd(stats + kTotComposited, nc);
+      atomicAdd(stats + kTotHitRays, nh);
+      if (nr) {
+        atomicAdd(stats + kTotTerminated, nr);
+        atomicAdd(stats + kTotTermSamples, nt);
+      }
+    }
+  }
   if (!valid) return;
   // the state carries into the next pass; after the last pass only a caller
   // that reads it again (keep: the training backward) stores it
@@ -446,6 +547,13 @@ __global__ __launch_bounds__(256) void composite(
 __global__ void zero_u32(uint32_t *p, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0u;
+}
+
+// FP32 expected depth -> the GRAY8 depth plane of the response.
+__global__ void depth_gray8_kernel(const float *__restrict__ depth, size_t n, float scale,
+                                   uint8_t *__restrict__ gray) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n) gray[i] = depth_gray8(depth[i], scale);
 }
 
 // ----------------------------------------------------------------- host
@@ -516,7 +624,7 @@ void scene_free(SceneDev &s) {
 }
 
 void render_work_free(RenderWork &w) {
-  for (DevBuf *b : {&w.views, &w.rays, &w.acc, &w.counts, &w.offsets, &w.ray_sh, &w.tinfo, &w.samples,
+  for (DevBuf *b : {&w.views, &w.rays, &w.acc, &w.counts, &w.offsets, &w.ray_sh, &w.tinfo, &w.gray, &w.samples,
                     &w.sample_out, &w.sample_pos, &w.feat, &w.out, &w.total})
     b->release();
   if (w.total_host) hipHostFree(w.total_host);
@@ -551,6 +659,10 @@ static OutPtrs out_ptrs(RenderWork &w) {
 }
 
 int render_setup(RenderWork &w, const nes_camera *views, int n_views, hipStream_t stream) {
+  // Until this render completes, the work area describes no views: a fetch
+  // after a failed setup or pass is refused instead of reading stale layouts.
+  w.n_views = 0;
+  w.out_px = 0;
   if (!views || n_views < 1) return NES_EINVAL;
   w.host_views.resize((size_t)n_views);
   uint64_t slots = 0, px = 0;
@@ -602,7 +714,7 @@ int render_setup(RenderWork &w, const nes_camera *views, int n_views, hipStream_
 
 int render_reserve_samples(RenderWork &w, int precision, size_t capacity) {
   if (capacity > 0xffffffffull) capacity = 0xffffffffull;
-  const size_t fbytes = precision == NES_PRECISION_FP32 ? sizeof(float4) : sizeof(uint2);
+  const size_t fbytes = precision != NES_PRECISION_MIXED ? sizeof(float4) : sizeof(uint2);
   NES_TRY(w.samples.reserve(sizeof(uint32_t) * (capacity + 1)));
   NES_TRY(w.sample_out.reserve(sizeof(float4) * (capacity + 1)));
   NES_TRY(w.sample_pos.reserve(sizeof(float4) * (capacity + 1)));
@@ -612,35 +724,65 @@ int render_reserve_samples(RenderWork &w, int precision, size_t capacity) {
   return NES_OK;
 }
 
+static MarchArgs march_args(RenderWork &w, const uint32_t *occ, int k0, int k1, bool first,
+                            uint32_t ray_cap, bool resume) {
+  MarchArgs m;
+  m.views = w.views.as<ViewDesc>();
+  m.n_views = w.n_views;
+  m.n_slots = w.n_slots;
+  m.occ = occ;
+  m.rays = w.rays.as<RayData>();
+  m.acc = w.acc.as<RayAcc>();
+  m.counts = w.counts.as<uint32_t>();
+  m.offsets = w.offsets.as<uint32_t>();
+  m.samples = w.samples.as<uint32_t>();
+  m.pos = w.sample_pos.as<float4>();
+  m.cursor = w.total.as<uint32_t>();
+  m.capacity = w.capacity;
+  m.cap = ray_cap;
+  m.k_begin = k0;
+  m.k_end = k1;
+  m.first = first ? 1 : 0;
+  m.packed = 0;
+  m.resume = resume ? 1 : 0;
+  m.ray_sh = nullptr;
+  m.tinfo = nullptr;
+  return m;
+}
+
 int render_launch_pass(const SceneDev &scene, RenderWork &w, int precision, int k0, int k1,
                        bool first, bool last, hipStream_t stream, hipEvent_t ev_eval_begin,
-                       hipEvent_t ev_eval_end, hipEvent_t ev_encode_end) {
+                       hipEvent_t ev_eval_end, hipEvent_t ev_encode_end, uint32_t ray_cap,
+                       bool resume) {
   const unsigned blocks = (unsigned)((w.n_slots + 255) / 256);
-  uint32_t *cursor = w.total.as<uint32_t>();
+  uint32_t *cursor = w.total.as<uint32_t>() + kTotCursor;
   const uint32_t cap = w.capacity;
   // a render in one pass keeps no per-ray state between march and composite
   uint2 *tinfo = first && last ? w.tinfo.as<uint2>() : nullptr;
-  zero_u32<<<1, 64, 0, stream>>>(cursor, 2);
-  march_emit<<<blocks, 256, 0, stream>>>(w.views.as<ViewDesc>(), w.n_views, w.n_slots, scene.occ,
-                                        w.rays.as<RayData>(), w.acc.as<RayAcc>(),
-                                        w.counts.as<uint32_t>(), w.offsets.as<uint32_t>(),
-                                        w.samples.as<uint32_t>(), w.sample_pos.as<float4>(),
-                                        cursor, cap, k0, k1, first ? 1 : 0,
-                                        precision == NES_PRECISION_MIXED ? 1 : 0,
-                                        precision == NES_PRECISION_MIXED ? w.ray_sh.as<uint2>()
-                                                                         : nullptr,
-                                        tinfo);
+  // the frame counters start with the first pass, the cursor with every pass
+  zero_u32<<<1, 64, 0, stream>>>(w.total.as<uint32_t>(), first ? kTotWords : 1);
+  MarchArgs m = march_args(w, scene.occ, k0, k1, first, ray_cap, resume);
+  m.packed = precision == NES_PRECISION_MIXED ? 1 : 0;
+  m.ray_sh = precision == NES_PRECISION_MIXED ? w.ray_sh.as<uint2>() : nullptr;
+  m.tinfo = tinfo;
+  march_emit<<<blocks, 256, 0, stream>>>(m);
   NES_HIP_TRY(hipGetLastError());
   if (ev_eval_begin) NES_HIP_TRY(hipEventRecord(ev_eval_begin, stream));
   // fixed grids with grid-stride loops: the sample count stays on the device
   const uint32_t bpp = part_blocks(cap);
-  if (precision == NES_PRECISION_FP32) {
+  const bool f32 = precision != NES_PRECISION_MIXED;
+  if (f32) {
     encode_parts<float><<<bpp * kParts, 256, 0, stream>>>(w.sample_pos.as<float4>(), cursor, cap,
                                                            scene.table_f32, w.feat.as<float4>(),
                                                            bpp);
     NES_HIP_TRY(hipGetLastError());
     if (ev_encode_end) NES_HIP_TRY(hipEventRecord(ev_encode_end, stream));
-    NES_TRY(launch_mlp_f32(scene.weights_f32, w, stream));
+    if (precision == NES_PRECISION_FP32_VALU)
+      mlp_fp32_valu<<<256 * 4, 256, 0, stream>>>(w.sample_pos.as<float4>(), w.feat.as<float4>(),
+                                                 cursor, cap, w.rays.as<RayData>(),
+                                                 scene.weights_f32, w.sample_out.as<float4>());
+    else
+      NES_TRY(launch_mlp_f32(scene.weights_f32, w, stream));
   } else {
     encode_parts<_Float16><<<bpp * kParts, 256, 0, stream>>>(w.sample_pos.as<uint2>(), cursor,
                                                               cap, scene.table_f16,
@@ -655,32 +797,117 @@ int render_launch_pass(const SceneDev &scene, RenderWork &w, int precision, int 
   NES_HIP_TRY(hipGetLastError());
   if (ev_eval_end) NES_HIP_TRY(hipEventRecord(ev_eval_end, stream));
   const OutPtrs optr = out_ptrs(w);
-  if (precision == NES_PRECISION_FP32)
+  uint32_t *stats = w.total.as<uint32_t>();
+  if (f32)
     composite<float4><<<blocks, 256, 0, stream>>>(
         w.n_slots, w.rays.as<RayData>(), w.acc.as<RayAcc>(), w.counts.as<uint32_t>(),
         w.offsets.as<uint32_t>(), w.samples.as<uint32_t>(), w.sample_out.as<float4>(), cursor, cap,
-        0, last ? 1 : 0, optr, tinfo);
+        0, last ? 1 : 0, optr, tinfo, stats);
   else
     composite<uint2><<<blocks, 256, 0, stream>>>(
         w.n_slots, w.rays.as<RayData>(), w.acc.as<RayAcc>(), w.counts.as<uint32_t>(),
         w.offsets.as<uint32_t>(), w.samples.as<uint32_t>(), w.sample_out.as<uint2>(), cursor, cap,
-        0, last ? 1 : 0, optr, tinfo);
+        0, last ? 1 : 0, optr, tinfo, stats);
   NES_HIP_TRY(hipGetLastError());
-  NES_HIP_TRY(hipMemcpyAsync(w.total_host, cursor, 4, hipMemcpyDeviceToHost, stream));
+  NES_HIP_TRY(hipMemcpyAsync(w.total_host + (first ? 0 : 1), cursor, 4, hipMemcpyDeviceToHost,
+                             stream));
+  if (last)
+    NES_HIP_TRY(hipMemcpyAsync(w.total_host + 2, stats + kTotComposited, 4 * (kTotWords - 1),
+                               hipMemcpyDeviceToHost, stream));
   return NES_OK;
+}
+
+int render_launch_frame(const SceneDev &scene, RenderWork &w, int precision, hipStream_t stream,
+                        const hipEvent_t *ev) {
+  w.total_host[1] = 0;  // no second pass unless launched below
+  const uint32_t cap1 = w.first_pass_cap;
+  auto rec = [&](int i) { return ev ? hipEventRecord(ev[i], stream) : hipSuccess; };
+  NES_HIP_TRY(rec(0));
+  if (cap1 >= (uint32_t)kMaxSteps) {
+    w.frame_passes = 1;
+    return render_launch_pass(scene, w, precision, 0, kMaxSteps, true, true, stream,
+                              ev ? ev[1] : nullptr, ev ? ev[3] : nullptr, ev ? ev[2] : nullptr);
+  }
+  // two passes, no host round trip between them: the second resumes every
+  // ray the first left above T = 1e-4 at the step after its last sample
+  w.frame_passes = 2;
+  NES_TRY(render_launch_pass(scene, w, precision, 0, kMaxSteps, true, false, stream,
+                             ev ? ev[1] : nullptr, ev ? ev[3] : nullptr, ev ? ev[2] : nullptr,
+                             cap1, false));
+  NES_HIP_TRY(rec(4));
+  return render_launch_pass(scene, w, precision, 0, kMaxSteps, false, true, stream,
+                            ev ? ev[5] : nullptr, ev ? ev[7] : nullptr, ev ? ev[6] : nullptr,
+                            kMaxSteps, true);
+}
+
+uint32_t render_frame_max_total(const RenderWork &w) {
+  return std::max(w.total_host[0], w.frame_passes > 1 ? w.total_host[1] : 0u);
+}
+
+int render_launch_bounded(const SceneDev &scene, RenderWork &w, int precision,
+                          hipStream_t stream) {
+  if (w.capacity < w.n_slots) {
+    set_last_error("sample buffers hold less than one sample per ray");
+    return NES_ENOMEM;
+  }
+  // each pass emits at most n_slots * cap <= capacity samples: no overflow
+  const uint32_t cap = std::min<uint32_t>(w.capacity / std::max<uint32_t>(w.n_slots, 1u),
+                                          kMaxSteps);
+  const int passes = (int)((kMaxSteps + cap - 1) / cap);
+  w.total_host[1] = 0;
+  w.frame_passes = (uint32_t)passes;
+  for (int p = 0; p < passes; ++p)
+    NES_TRY(render_launch_pass(scene, w, precision, 0, kMaxSteps, p == 0, p == passes - 1, stream,
+                               nullptr, nullptr, nullptr, cap, p > 0));
+  return NES_OK;
+}
+
+bool render_growth_fits(const RenderWork &w, int precision, size_t capacity) {
+  if (!w.budget) return true;
+  const size_t cur = w.samples.bytes + w.sample_out.bytes + w.sample_pos.bytes + w.feat.bytes;
+  return work_device_bytes(w) - cur + sample_buffer_bytes(capacity, precision) <= w.budget;
+}
+
+size_t sample_buffer_bytes(size_t capacity, int precision) {
+  const size_t c = capacity + 1, fb = precision != NES_PRECISION_MIXED ? sizeof(float4) : sizeof(uint2);
+  size_t t = 0;
+  // the buffers of render_reserve_samples, with DevBuf's growth slack
+  for (size_t need : {sizeof(uint32_t) * c, sizeof(float4) * c, sizeof(float4) * c, fb * kParts * c})
+    t += need + need / 8 + 256;
+  return t;
+}
+
+// Early-termination policy for the next frame of this work area.  A single
+// pass evaluates every occupied sample; when more than 5 % of them lie behind
+// the point where T fell below 1e-4, the next frame runs two passes, the first
+// giving each ray as many samples as the terminated rays needed on average
+// (rounded up).  Two-pass frames go back to one pass when fewer than 5 % of
+// the rays with samples terminate early.
+FrameCounts render_frame_finish(RenderWork &w) {
+  FrameCounts c;
+  c.passes = w.frame_passes;
+  c.evaluated = w.total_host[6];
+  c.composited = w.total_host[2];
+  c.hit_rays = w.total_host[3];
+  c.terminated = w.total_host[4];
+  const uint64_t term_samples = w.total_host[5];
+  const bool wasteful = c.evaluated > c.composited && (c.evaluated - c.composited) * 20 > c.evaluated;
+  const bool terminating = c.hit_rays > 0 && c.terminated * 20 > c.hit_rays;
+  if ((c.passes == 1 && wasteful) || (c.passes > 1 && terminating)) {
+    const uint64_t m = (term_samples + c.terminated - 1) / std::max<uint64_t>(c.terminated, 1);
+    w.first_pass_cap = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(m, 1), kMaxSteps - 1);
+  } else {
+    w.first_pass_cap = kMaxSteps;
+  }
+  return c;
 }
 
 // Single-stage launchers reused by the training step (rays and compositing
 // state already initialised by the caller, one output index per ray slot).
 int launch_march_emit(const uint32_t *occ, RenderWork &w, int k0, int k1, hipStream_t stream) {
   const unsigned blocks = (unsigned)((w.n_slots + 255) / 256);
-  uint32_t *cursor = w.total.as<uint32_t>();
-  zero_u32<<<1, 64, 0, stream>>>(cursor, 2);
-  march_emit<<<blocks, 256, 0, stream>>>(w.views.as<ViewDesc>(), w.n_views, w.n_slots, occ,
-                                        w.rays.as<RayData>(), w.acc.as<RayAcc>(),
-                                        w.counts.as<uint32_t>(), w.offsets.as<uint32_t>(),
-                                        w.samples.as<uint32_t>(), w.sample_pos.as<float4>(),
-                                        cursor, w.capacity, k0, k1, 0, 0, nullptr, nullptr);
+  zero_u32<<<1, 64, 0, stream>>>(w.total.as<uint32_t>(), kTotWords);
+  march_emit<<<blocks, 256, 0, stream>>>(march_args(w, occ, k0, k1, false, kMaxSteps, false));
   NES_HIP_TRY(hipGetLastError());
   return NES_OK;
 }
@@ -700,7 +927,7 @@ int launch_composite(RenderWork &w, bool last, hipStream_t stream) {
                                                w.counts.as<uint32_t>(), w.offsets.as<uint32_t>(),
                                                w.samples.as<uint32_t>(), w.sample_out.as<float4>(),
                                                w.total.as<uint32_t>(), w.capacity, 1, last ? 1 : 0,
-                                               out_ptrs(w), nullptr);
+                                               out_ptrs(w), nullptr, nullptr);
   NES_HIP_TRY(hipGetLastError());
   return NES_OK;
 }
@@ -716,7 +943,7 @@ int render_prepare_capacity(RenderWork &w, int precision) {
 
 size_t work_device_bytes(const RenderWork &w) {
   size_t b = 0;
-  for (const DevBuf *d : {&w.views, &w.rays, &w.acc, &w.counts, &w.offsets, &w.ray_sh, &w.tinfo, &w.samples,
+  for (const DevBuf *d : {&w.views, &w.rays, &w.acc, &w.counts, &w.offsets, &w.ray_sh, &w.tinfo, &w.gray, &w.samples,
                           &w.sample_out, &w.sample_pos, &w.feat, &w.out, &w.total})
     b += d->bytes;
   return b;
@@ -726,7 +953,7 @@ size_t work_device_bytes(const RenderWork &w) {
 // render starts from nothing, and report NES_ENOMEM.
 static int check_budget(RenderWork &w) {
   if (w.budget && work_device_bytes(w) > w.budget) {
-    for (DevBuf *d : {&w.views, &w.rays, &w.acc, &w.counts, &w.offsets, &w.ray_sh, &w.tinfo, &w.samples,
+    for (DevBuf *d : {&w.views, &w.rays, &w.acc, &w.counts, &w.offsets, &w.ray_sh, &w.tinfo, &w.gray, &w.samples,
                       &w.sample_out, &w.sample_pos, &w.feat, &w.out, &w.total})
       d->release();
     w.capacity = 0;
@@ -738,60 +965,136 @@ static int check_budget(RenderWork &w) {
   return NES_OK;
 }
 
+static int render_enqueue_inner(const SceneDev &scene, RenderWork &w, int precision,
+                                const nes_camera *views, int n_views, int steps_per_pass,
+                                hipStream_t stream, nes_render_stats *stats);
+
 int render_enqueue(const SceneDev &scene, RenderWork &w, int precision,
                    const nes_camera *views, int n_views, int steps_per_pass,
                    hipStream_t stream, nes_render_stats *stats) {
-  if (precision != NES_PRECISION_FP32 && precision != NES_PRECISION_MIXED) return NES_EINVAL;
-  if (steps_per_pass < 0) return NES_EINVAL;
-  const int spp = steps_per_pass == 0 ? kMaxSteps : std::min(steps_per_pass, kMaxSteps);
+  const int rc = render_enqueue_inner(scene, w, precision, views, n_views, steps_per_pass, stream,
+                                      stats);
+  if (rc != NES_OK) {
+    // a failed render leaves nothing to fetch
+    w.n_views = 0;
+    w.out_px = 0;
+  }
+  return rc;
+}
+
+static bool valid_precision(int p) {
+  return p == NES_PRECISION_FP32 || p == NES_PRECISION_MIXED || p == NES_PRECISION_FP32_VALU;
+}
+
+// Stage times of one pass from its four events and the event that follows it.
+static void add_pass_times(const hipEvent_t *ev, hipEvent_t next, double t[4]) {
+  float a = 0, b = 0, c = 0, d = 0;
+  hipEventElapsedTime(&a, ev[0], ev[1]);
+  hipEventElapsedTime(&b, ev[1], ev[2]);
+  hipEventElapsedTime(&c, ev[2], ev[3]);
+  hipEventElapsedTime(&d, ev[3], next);
+  t[0] += a;  // march
+  t[1] += b;  // encode
+  t[2] += c;  // MLPs
+  t[3] += d;  // composite
+}
+
+static int render_enqueue_inner(const SceneDev &scene, RenderWork &w, int precision,
+                                const nes_camera *views, int n_views, int steps_per_pass,
+                                hipStream_t stream, nes_render_stats *stats) {
+  if (!valid_precision(precision) || steps_per_pass < 0) return NES_EINVAL;
   NES_TRY(render_setup(w, views, n_views, stream));
   NES_TRY(render_prepare_capacity(w, precision));
   NES_TRY(check_budget(w));
-  double ms_march = 0, ms_eval = 0, ms_comp = 0, ms_enc = 0;
-  uint64_t n_samples = 0;
+  double t[4] = {0, 0, 0, 0};  // march, encode, MLPs, composite
+  FrameCounts fc;
+  hipEvent_t *ev = w.ev + 1, end = w.ev[9];
   NES_HIP_TRY(hipEventRecord(w.ev[0], stream));
-  const int passes = (kMaxSteps + spp - 1) / spp;
-  for (int p = 0; p < passes; ++p) {
-    const int k0 = p * spp, k1 = std::min(kMaxSteps, k0 + spp);
+  if (steps_per_pass == 0 || steps_per_pass >= kMaxSteps) {
+    // whole frame: one pass, or two when the previous frame of this work area
+    // evaluated samples behind terminated rays (steps_per_pass 0)
+    if (steps_per_pass) w.first_pass_cap = kMaxSteps;
+    bool bounded = false;
     for (int attempt = 0;; ++attempt) {
-      NES_HIP_TRY(hipEventRecord(w.ev[1], stream));
-      NES_TRY(render_launch_pass(scene, w, precision, k0, k1, p == 0, p == passes - 1, stream,
-                                 w.ev[2], w.ev[3], w.ev[5]));
-      NES_HIP_TRY(hipEventRecord(w.ev[4], stream));
-      NES_HIP_TRY(hipEventSynchronize(w.ev[4]));
-      const uint32_t n = w.total_host[0];
-      if (n <= w.capacity) {
-        n_samples += n;
-        break;
+      if (bounded) {
+        NES_HIP_TRY(hipEventRecord(ev[0], stream));
+        NES_TRY(render_launch_bounded(scene, w, precision, stream));
+      } else {
+        NES_TRY(render_launch_frame(scene, w, precision, stream, ev));
       }
+      NES_HIP_TRY(hipEventRecord(end, stream));
+      NES_HIP_TRY(hipEventSynchronize(end));
+      const uint32_t n = render_frame_max_total(w);
+      if (bounded || n <= w.capacity) break;
       if (attempt > 0) {
         set_last_error("sample capacity still exceeded after regrowth");
         return NES_ENOMEM;
       }
-      NES_TRY(render_reserve_samples(w, precision, (size_t)n + n / 8 + 1024));
-      NES_TRY(check_budget(w));
+      // grow when the budget allows it, else render within the buffers
+      const size_t want = (size_t)n + n / 8 + 1024;
+      if (render_growth_fits(w, precision, want))
+        NES_TRY(render_reserve_samples(w, precision, want));
+      else
+        bounded = true;
     }
-    float a = 0, b = 0, c = 0, e = 0;
-    hipEventElapsedTime(&a, w.ev[1], w.ev[2]);
-    hipEventElapsedTime(&b, w.ev[2], w.ev[3]);
-    hipEventElapsedTime(&c, w.ev[3], w.ev[4]);
-    hipEventElapsedTime(&e, w.ev[2], w.ev[5]);
-    ms_march += a;
-    ms_eval += b;
-    ms_comp += c;
-    ms_enc += e;
+    if (bounded) {
+      float a = 0;
+      hipEventElapsedTime(&a, ev[0], end);
+      t[1] += a;  // no per-stage events over bounded passes
+    } else {
+      add_pass_times(ev, w.frame_passes > 1 ? ev[4] : end, t);
+      if (w.frame_passes > 1) add_pass_times(ev + 4, end, t);
+    }
+    fc = render_frame_finish(w);
+  } else {
+    // fixed windows of steps_per_pass march steps, one host round trip each
+    const int spp = steps_per_pass;
+    const int passes = (kMaxSteps + spp - 1) / spp;
+    w.frame_passes = 1;
+    for (int p = 0; p < passes; ++p) {
+      const int k0 = p * spp, k1 = std::min(kMaxSteps, k0 + spp);
+      uint32_t n = 0;
+      for (int attempt = 0;; ++attempt) {
+        NES_HIP_TRY(hipEventRecord(ev[0], stream));
+        NES_TRY(render_launch_pass(scene, w, precision, k0, k1, p == 0, p == passes - 1, stream,
+                                   ev[1], ev[3], ev[2]));
+        NES_HIP_TRY(hipEventRecord(end, stream));
+        NES_HIP_TRY(hipEventSynchronize(end));
+        n = w.total_host[p == 0 ? 0 : 1];
+        if (n <= w.capacity) break;
+        if (attempt > 0) {
+          set_last_error("sample capacity still exceeded after regrowth");
+          return NES_ENOMEM;
+        }
+        const size_t want = (size_t)n + n / 8 + 1024;
+        if (!render_growth_fits(w, precision, want)) {
+          set_last_error("a pass's samples exceed the context's HBM budget");
+          return NES_ENOMEM;
+        }
+        NES_TRY(render_reserve_samples(w, precision, want));
+      }
+      fc.evaluated += n;
+      add_pass_times(ev, end, t);
+    }
+    fc.composited = w.total_host[2];
+    fc.hit_rays = w.total_host[3];
+    fc.terminated = w.total_host[4];
+    fc.passes = (uint32_t)passes;
   }
-  float t = 0;
-  hipEventElapsedTime(&t, w.ev[0], w.ev[4]);
+  float total = 0;
+  hipEventElapsedTime(&total, w.ev[0], end);
   if (stats) {
-    stats->ms_encode = ms_enc;
-    stats->ms_mlp = ms_eval - ms_enc;
     stats->rays = w.out_px;
-    stats->samples = n_samples;
-    stats->ms_total = t;
-    stats->ms_eval = ms_eval;
-    stats->ms_march = ms_march;
-    stats->ms_composite = ms_comp;
+    stats->samples = fc.evaluated;
+    stats->ms_total = total;
+    stats->ms_march = t[0];
+    stats->ms_encode = t[1];
+    stats->ms_mlp = t[2];
+    stats->ms_eval = t[1] + t[2];
+    stats->ms_composite = t[3];
+    stats->composited = fc.composited;
+    stats->terminated_rays = fc.terminated;
+    stats->passes = fc.passes;
   }
   return NES_OK;
 }
@@ -800,6 +1103,10 @@ int render_fetch(RenderWork &w, int view, const nes_view_out *out, hipStream_t s
   if (!out || view < 0 || (uint32_t)view >= w.n_views) return NES_EINVAL;
   const ViewDesc &v = w.host_views[(size_t)view];
   const size_t px = (size_t)v.width * v.height, base = v.px_base;
+  if (base + px > w.out_px) {
+    set_last_error("view lies outside the last render's outputs");
+    return NES_EINVAL;
+  }
   const OutLayout L = out_layout(w.out_px);
   const uint8_t *o = w.out.as<uint8_t>();
   if (out->rgb24)
@@ -810,8 +1117,28 @@ int render_fetch(RenderWork &w, int view, const nes_view_out *out, hipStream_t s
     NES_HIP_TRY(hipMemcpyAsync(out->rgb_f32, o + L.rgbf + 12 * base, 12 * px, hipMemcpyDeviceToHost, stream));
   if (out->n_samples)
     NES_HIP_TRY(hipMemcpyAsync(out->n_samples, o + L.nsamp + 2 * base, 2 * px, hipMemcpyDeviceToHost, stream));
+  if (out->depth_gray8) {
+    NES_TRY(w.gray.reserve(px));
+    depth_gray8_kernel<<<(unsigned)((px + 255) / 256), 256, 0, stream>>>(
+        reinterpret_cast<const float *>(o + L.depth) + base, px, depth_scale(out->depth_far),
+        w.gray.as<uint8_t>());
+    NES_HIP_TRY(hipGetLastError());
+    NES_HIP_TRY(hipMemcpyAsync(out->depth_gray8, w.gray.ptr, px, hipMemcpyDeviceToHost, stream));
+  }
   NES_HIP_TRY(hipStreamSynchronize(stream));
   return NES_OK;
+}
+
+std::shared_ptr<SceneDev> new_scene() {
+  return std::shared_ptr<SceneDev>(new SceneDev(), [](SceneDev *p) {
+    scene_free(*p);
+    delete p;
+  });
+}
+
+std::shared_ptr<SceneDev> ctx_scene(nes_ctx *ctx) {
+  std::lock_guard<std::mutex> lk(ctx->scene_mu);
+  return ctx->scene;
 }
 
 }  // namespace nes
@@ -826,10 +1153,10 @@ int nes_ctx_create(const nes_scene_desc *scene, uint64_t hbm_budget_bytes, nes_c
   nes_ctx *c = new nes_ctx();
   c->budget = hbm_budget_bytes ? hbm_budget_bytes : (16ull << 30);
   c->work.budget = c->budget;
-  int rc = scene_upload(c->scene, scene);
+  c->scene = new_scene();
+  int rc = scene_upload(*c->scene, scene);
   if (rc == NES_OK && hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) rc = NES_EIO;
   if (rc != NES_OK) {
-    scene_free(c->scene);
     delete c;
     return rc;
   }
@@ -843,26 +1170,37 @@ int nes_ctx_destroy(nes_ctx *ctx) {
     std::lock_guard<std::mutex> lk(ctx->mu);
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
     render_work_free(ctx->work);
-    scene_free(ctx->scene);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
   }
-  delete ctx;
+  delete ctx;  // frees the scene copies (servers on it are destroyed first)
   return NES_OK;
 }
 
 int nes_ctx_update_scene(nes_ctx *ctx, const nes_scene_desc *scene) {
-  if (!ctx) return NES_EINVAL;
-  std::lock_guard<std::mutex> lk(ctx->mu);
-  NES_HIP_TRY(hipStreamSynchronize(ctx->stream));
-  return scene_upload(ctx->scene, scene);
+  if (!ctx || !scene || !scene->table || !scene->weights || !scene->occupancy) return NES_EINVAL;
+  std::shared_ptr<SceneDev> next;
+  {
+    // reuse the previous copy once no render or batch holds it any more
+    std::lock_guard<std::mutex> lk(ctx->scene_mu);
+    if (ctx->spare && ctx->spare.use_count() == 1) next = ctx->spare;
+    ctx->spare.reset();
+  }
+  if (!next) next = new_scene();
+  // the copy being written is referenced by no launched work
+  NES_TRY(scene_upload(*next, scene));
+  std::lock_guard<std::mutex> lk(ctx->scene_mu);
+  ctx->spare = ctx->scene;
+  ctx->scene = next;
+  return NES_OK;
 }
 
 int nes_render_device(nes_ctx *ctx, int precision, const nes_camera *views, int n_views,
                       int steps_per_pass, nes_render_stats *stats) {
   if (!ctx) return NES_EINVAL;
   std::lock_guard<std::mutex> lk(ctx->mu);
-  return render_enqueue(ctx->scene, ctx->work, precision, views, n_views, steps_per_pass,
-                        ctx->stream, stats);
+  const std::shared_ptr<SceneDev> sc = ctx_scene(ctx);
+  return render_enqueue(*sc, ctx->work, precision, views, n_views, steps_per_pass, ctx->stream,
+                        stats);
 }
 
 int nes_render_fetch(nes_ctx *ctx, int view, const nes_view_out *out) {
@@ -875,8 +1213,9 @@ int nes_render(nes_ctx *ctx, int precision, const nes_camera *views, int n_views
                const nes_view_out *outs, int steps_per_pass, nes_render_stats *stats) {
   if (!ctx || !outs) return NES_EINVAL;
   std::lock_guard<std::mutex> lk(ctx->mu);
-  NES_TRY(render_enqueue(ctx->scene, ctx->work, precision, views, n_views, steps_per_pass,
-                         ctx->stream, stats));
+  const std::shared_ptr<SceneDev> sc = ctx_scene(ctx);
+  NES_TRY(render_enqueue(*sc, ctx->work, precision, views, n_views, steps_per_pass, ctx->stream,
+                         stats));
   for (int i = 0; i < n_views; ++i) NES_TRY(render_fetch(ctx->work, i, &outs[i], ctx->stream));
   return NES_OK;
 }

@@ -4,23 +4,30 @@
 // Replaces the reference's request loop: one socket_client_thread per
 // renderer connection, each with a single blocking FrameRequest in flight
 // (src/server.cpp:116-212), feeding process_frame_thread's RGB24 -> YUV420P
-// conversion (src/encode.cpp:42-119).  Here:
+// conversion of the scene and the GRAY8 depth (src/encode.cpp:42-119,
+// include/base/video/rendered_frame.h:24-33).  Here:
 //   * any number of client threads submit stereo requests (nes_server_submit);
 //     like CameraManager (src/base/camera_manager.cc:33-69), only a client's
 //     latest pose matters: a newer request supersedes one still waiting for
 //     admission;
 //   * one scheduler thread forms batches of waiting requests as soon as a
 //     batch slot is free.  A slot owns one HIP stream and the device buffers
-//     of one batch; the number of slots is what the HBM budget affords, so
-//     admission is by memory.  Batches on different slots overlap on the
-//     device, and a request arriving while others render joins the next free
-//     slot without waiting for them (continuous batching);
-//   * each batch is one asynchronous launch sequence: the render pass (march,
-//     encode, MLPs, composite -- the sample count never leaves the device),
-//     frame prep of every view straight from the device RGB24, and an async
-//     copy of the YUV420P planes into pinned host memory.  The scheduler polls
-//     completion events, so it never blocks on one batch while another could
-//     progress.
+//     of one batch; the number of slots is what the HBM budget affords, and
+//     each slot renders within its share of the budget (a batch whose samples
+//     would not fit is rendered again in bounded-memory passes instead of
+//     growing), so admission is by memory.  Batches on different slots
+//     overlap on the device, and a request arriving while others render joins
+//     the next free slot without waiting for them (continuous batching);
+//   * each batch is one asynchronous launch sequence on the scene the context
+//     held when the batch was formed: the render (march, encode, MLPs,
+//     composite; one pass, or two when the slot's previous batch evaluated
+//     samples behind terminated rays -- the sample count never leaves the
+//     device), frame prep of every view straight from the device RGB24 (and,
+//     for depth-stream servers, the depth plane straight from the device
+//     FP32 depth), and async copies of the YUV420P planes into host memory:
+//     the server's pinned planes or the caller's own (nes_server_submit_into).
+//     The scheduler polls completion events, so it never blocks on one batch
+//     while another could progress.
 // All per-ray work is independent of batch composition, so a request's frame
 // is bit-identical alone or in any mix.
 #include <algorithm>
@@ -59,6 +66,10 @@ class PinnedPool {
       for (uint8_t *p : kv.second) hipHostFree(p);
   }
   int get(size_t bytes, PinnedBlock *out) {
+    if (!bytes) {
+      *out = PinnedBlock();
+      return NES_OK;
+    }
     {
       std::lock_guard<std::mutex> lk(mu_);
       auto &v = free_[bytes];
@@ -92,27 +103,47 @@ struct Request {
   int n_views = 0;
   nes_camera views[kMaxViews];
   Clock::time_point arrival, admitted, done;
+  // arrivals of the requests this one superseded (served by this frame)
+  std::vector<Clock::time_point> replaced;
   ReqState state = ReqState::kWaiting;
   int status = NES_OK;
-  PinnedBlock yuv;  // per view: Y then U then V, tightly packed
-  size_t view_off[kMaxViews] = {};
-  uint64_t rays = 0, samples = 0;
+  // Output planes per view: scene (planes[v]) and depth (dplanes[v]), either
+  // the caller's or laid out tightly in `yuv` (pinned).
+  PinnedBlock yuv;
+  bool caller = false;
+  nes_frame_planes planes[kMaxViews] = {}, dplanes[kMaxViews] = {};
+  uint64_t rays = 0, samples = 0, batch = 0;
+  uint32_t batch_requests = 0, batch_passes = 0;
 };
 
 static size_t yuv_bytes(const nes_camera &c) {
   return (size_t)c.width * c.height + 2 * (size_t)(c.width / 2) * (c.height / 2);
 }
 
+// Tight YUV420P planes of a w x h frame starting at p.
+static nes_frame_planes tight_planes(uint8_t *p, const nes_camera &c) {
+  nes_frame_planes f;
+  f.data[0] = p;
+  f.data[1] = p + (size_t)c.width * c.height;
+  f.data[2] = f.data[1] + (size_t)(c.width / 2) * (c.height / 2);
+  f.linesize[0] = (int32_t)c.width;
+  f.linesize[1] = f.linesize[2] = (int32_t)(c.width / 2);
+  return f;
+}
+
 struct Slot {
   hipStream_t stream = nullptr;
   RenderWork work;
-  DevBuf yuv;
-  DevBuf ovl_place;                    // text overlay placements (device)
+  DevBuf yuv;                            // device YUV420P planes of the batch
+  DevBuf ovl_place;                      // text overlay placements (device)
   std::vector<GlyphPlacement> ovl_host;  // ... and their host copy
   hipEvent_t ev_done = nullptr;
   std::vector<std::shared_ptr<Request>> reqs;
   std::vector<nes_camera> views;
-  int stage = 0;  // 0 idle, 2 in flight
+  std::shared_ptr<SceneDev> scene;  // the scene this batch renders
+  uint64_t batch_id = 0;
+  bool bounded = false;             // rendered in bounded-memory passes
+  int stage = 0;                    // 0 idle, 2 in flight
   Clock::time_point launched;
 };
 
@@ -129,13 +160,14 @@ struct nes_server {
   std::deque<std::shared_ptr<Request>> waiting;
   std::map<uint64_t, std::shared_ptr<Request>> tickets;
   std::map<uint32_t, std::shared_ptr<Request>> client_waiting;
-  uint64_t next_ticket = 1;
-  bool stop = false;
+  uint64_t next_ticket = 1, next_batch = 1;
+  bool stop = false, hold = false;
   std::thread scheduler;
   PinnedPool pinned;
   // statistics
-  std::vector<double> latencies_ms;
+  std::vector<double> latencies_ms, latencies_all_ms;
   uint64_t completed = 0, superseded = 0, failed = 0, rays = 0, samples = 0, batches = 0;
+  uint64_t composited = 0, bounded_batches = 0;
   Clock::time_point first_arrival, last_done;
   bool any_arrival = false;
   int last_error = NES_OK;
@@ -145,26 +177,44 @@ namespace nes {
 
 // Device bytes one batch slot holds for max_rays ray slots, as the render
 // allocates them: per ray slot the ray state (RayData, RayAcc), counts and
-// offsets, the FP16 SH of the ray, its single-pass record, per-pixel outputs (RGB24, depth, float RGB, sample count) and the
-// YUV420P frame; per sample (the initial buffer holds 4 per ray slot, at
+// offsets, the FP16 SH of the ray, its single-pass record, per-pixel outputs
+// (RGB24, depth, float RGB, sample count) and the YUV420P frame (twice with
+// the depth stream); per sample (the initial buffer holds 4 per ray slot, at
 // least 2^20) the record, position, MLP output and the 8 feature planes;
-// plus the 1/8 growth slack of every device buffer.  A batch that marches
-// more samples grows its buffers past this (up to 128 per ray).
-static size_t slot_bytes(uint32_t max_rays, int precision) {
+// plus the 1/8 growth slack of every device buffer.  A batch never grows its
+// sample buffers past the slot's share of the budget (it renders in bounded
+// passes instead).
+static size_t slot_bytes(uint32_t max_rays, int precision, bool depth) {
   const size_t r = max_rays;
   const size_t cap = std::max<size_t>(r * 4, (size_t)1 << 20);
-  const size_t feat = precision == NES_PRECISION_FP32 ? sizeof(float4) : sizeof(uint2);
-  const size_t per_ray = sizeof(RayData) + sizeof(RayAcc) + 8 + 32 + 8 + (3 + 4 + 12 + 2) + 2;
+  const size_t feat = precision != NES_PRECISION_MIXED ? sizeof(float4) : sizeof(uint2);
+  const size_t per_ray = sizeof(RayData) + sizeof(RayAcc) + 8 + 32 + 8 + (3 + 4 + 12 + 2) +
+                         (depth ? 4 : 2);
   const size_t per_sample = 4 + sizeof(float4) + sizeof(float4) + feat * 8;  // 8 XCD part planes
   return (r * per_ray + cap * per_sample) / 8 * 9 + ((size_t)1 << 20);
+}
+
+// Device-to-host copy of one view's YUV420P planes (tight on the device)
+// into the request's planes.
+static int copy_planes(const nes_frame_planes &dst, const uint8_t *src, const nes_camera &c,
+                       hipStream_t stream) {
+  const nes_frame_planes s = tight_planes(const_cast<uint8_t *>(src), c);
+  const size_t w[3] = {c.width, c.width / 2, c.width / 2};
+  const size_t h[3] = {c.height, c.height / 2, c.height / 2};
+  for (int p = 0; p < 3; ++p)
+    NES_HIP_TRY(hipMemcpy2DAsync(dst.data[p], (size_t)dst.linesize[p], s.data[p],
+                                 (size_t)s.linesize[p], w[p], h[p], hipMemcpyDeviceToHost,
+                                 stream));
+  return NES_OK;
 }
 
 // Launch frame prep of every view of a finished render and the D2H copies.
 static int launch_output(nes_server *s, Slot &sl) {
   RenderWork &w = sl.work;
   const OutLayout L = out_layout(w.out_px);
+  const bool depth = s->cfg.depth_stream != 0;
   size_t yuv_total = 0;
-  for (const nes_camera &c : sl.views) yuv_total += yuv_bytes(c);
+  for (const nes_camera &c : sl.views) yuv_total += yuv_bytes(c) * (depth ? 2 : 1);
   NES_TRY(sl.yuv.reserve(yuv_total + 256));
   if (s->cfg.font && s->cfg.overlay_flags) {
     // the strings of every view, stamped by one launch before frame prep
@@ -178,22 +228,30 @@ static int launch_output(nes_server *s, Slot &sl) {
     NES_TRY(overlay_launch(s->cfg.font, sl.ovl_place, sl.ovl_host, w.out.as<uint8_t>(),
                            sl.stream));
   }
+  const float scale = depth_scale(s->cfg.depth_far);
   size_t off = 0;
   size_t vi = 0;
   for (auto &rq : sl.reqs) {
     for (int v = 0; v < rq->n_views; ++v, ++vi) {
       const nes_camera &c = sl.views[vi];
       const ViewDesc &vd = w.host_views[vi];
+      const int cw = (int)c.width, ch = (int)c.height;
       const uint8_t *rgb = w.out.as<uint8_t>() + L.rgb24 + 3 * (size_t)vd.px_base;
-      uint8_t *y = sl.yuv.as<uint8_t>() + off;
-      uint8_t *u = y + (size_t)c.width * c.height;
-      uint8_t *vv = u + (size_t)(c.width / 2) * (c.height / 2);
-      NES_TRY(launch_frameprep(rgb, nullptr, nullptr, nullptr, (int)c.width, (int)c.height, 1,
-                               0, 0, y, (int)c.width, 0, u, (int)c.width / 2, 0, vv,
-                               (int)c.width / 2, 0, s->cfg.sws_mode, sl.stream));
-      NES_HIP_TRY(hipMemcpyAsync(rq->yuv.ptr + rq->view_off[v], y, yuv_bytes(c),
-                                 hipMemcpyDeviceToHost, sl.stream));
+      const nes_frame_planes d = tight_planes(sl.yuv.as<uint8_t>() + off, c);
+      NES_TRY(launch_frameprep(rgb, nullptr, nullptr, nullptr, cw, ch, 1, 0, 0, d.data[0], cw, 0,
+                               d.data[1], cw / 2, 0, d.data[2], cw / 2, 0, s->cfg.sws_mode,
+                               sl.stream));
+      NES_TRY(copy_planes(rq->planes[v], d.data[0], c, sl.stream));
       off += yuv_bytes(c);
+      if (depth) {
+        const float *dep = reinterpret_cast<const float *>(w.out.as<uint8_t>() + L.depth) +
+                           vd.px_base;
+        const nes_frame_planes dd = tight_planes(sl.yuv.as<uint8_t>() + off, c);
+        NES_TRY(launch_depth_yuv(dep, cw, ch, scale, nullptr, dd.data[0], cw, dd.data[1], cw / 2,
+                                 dd.data[2], cw / 2, s->cfg.sws_mode, sl.stream));
+        NES_TRY(copy_planes(rq->dplanes[v], dd.data[0], c, sl.stream));
+        off += yuv_bytes(c);
+      }
     }
   }
   return NES_OK;
@@ -223,7 +281,8 @@ static void form_batch(nes_server *s, Slot &sl) {
 
 static void finish_batch(nes_server *s, Slot &sl, int status) {
   const Clock::time_point now = Clock::now();
-  const uint64_t n = sl.work.total_host ? sl.work.total_host[0] : 0;
+  FrameCounts fc;
+  if (status == NES_OK) fc = render_frame_finish(sl.work);
   uint64_t batch_rays = 0;
   for (auto &rq : sl.reqs)
     for (int v = 0; v < rq->n_views; ++v) batch_rays += (uint64_t)rq->views[v].width * rq->views[v].height;
@@ -235,28 +294,38 @@ static void finish_batch(nes_server *s, Slot &sl, int status) {
     uint64_t r = 0;
     for (int v = 0; v < rq->n_views; ++v) r += (uint64_t)rq->views[v].width * rq->views[v].height;
     rq->rays = r;
-    rq->samples = batch_rays ? (uint64_t)((double)n * (double)r / (double)batch_rays) : 0;
+    rq->samples = batch_rays ? (uint64_t)((double)fc.evaluated * (double)r / (double)batch_rays) : 0;
+    rq->batch = sl.batch_id;
+    rq->batch_requests = (uint32_t)sl.reqs.size();
+    rq->batch_passes = fc.passes;
     if (status == NES_OK) {
       s->completed++;
       s->rays += r;
-      s->latencies_ms.push_back(ms_between(rq->arrival, now));
+      const double lat = ms_between(rq->arrival, now);
+      s->latencies_ms.push_back(lat);
+      s->latencies_all_ms.push_back(lat);
+      for (const Clock::time_point &a : rq->replaced) s->latencies_all_ms.push_back(ms_between(a, now));
       s->last_done = now;
     } else {
       s->failed++;
     }
   }
-  s->samples += n;
+  s->samples += fc.evaluated;
+  s->composited += fc.composited;
+  s->bounded_batches += sl.bounded ? 1 : 0;
   s->batches++;
   sl.reqs.clear();
+  sl.scene.reset();
   sl.stage = 0;
   s->cv_done.notify_all();
 }
 
-// Launch the whole batch (render pass, frame prep, D2H) and its done event.
+// Launch the whole batch (render, frame prep, D2H) and its done event.
 static int launch_batch(nes_server *s, Slot &sl) {
-  const SceneDev &scene = s->ctx->scene;
-  NES_TRY(render_launch_pass(scene, sl.work, s->cfg.precision, 0, kMaxSteps, true, true,
-                             sl.stream, nullptr, nullptr, nullptr));
+  if (sl.bounded)
+    NES_TRY(render_launch_bounded(*sl.scene, sl.work, s->cfg.precision, sl.stream));
+  else
+    NES_TRY(render_launch_frame(*sl.scene, sl.work, s->cfg.precision, sl.stream));
   NES_TRY(launch_output(s, sl));
   NES_HIP_TRY(hipEventRecord(sl.ev_done, sl.stream));
   return NES_OK;
@@ -270,10 +339,16 @@ static void scheduler_loop(nes_server *s) {
       Slot &sl = *slp;
       if (sl.stage != 2 || hipEventQuery(sl.ev_done) != hipSuccess) continue;
       progressed = true;
-      const uint32_t n = sl.work.total_host[0];
-      if (n > sl.work.capacity) {
-        // the pass dropped samples beyond capacity: grow and run it again
-        int rc = render_reserve_samples(sl.work, s->cfg.precision, (size_t)n + n / 8 + 1024);
+      const uint32_t n = render_frame_max_total(sl.work);
+      if (!sl.bounded && n > sl.work.capacity) {
+        // the frame dropped samples beyond capacity: grow within the slot's
+        // share of the budget and run it again, or run it in bounded passes
+        const size_t want = (size_t)n + n / 8 + 1024;
+        int rc = NES_OK;
+        if (render_growth_fits(sl.work, s->cfg.precision, want))
+          rc = render_reserve_samples(sl.work, s->cfg.precision, want);
+        else
+          sl.bounded = true;
         if (rc == NES_OK) rc = launch_batch(s, sl);
         if (rc == NES_OK) continue;
         s->last_error = rc;
@@ -288,10 +363,13 @@ static void scheduler_loop(nes_server *s) {
       std::unique_lock<std::mutex> lk(s->mu);
       for (auto &slp : s->slots) {
         Slot &sl = *slp;
-        if (sl.stage != 0 || s->waiting.empty()) continue;
+        if (sl.stage != 0 || s->waiting.empty() || s->hold) continue;
         form_batch(s, sl);
         if (sl.reqs.empty()) continue;
+        sl.batch_id = s->next_batch++;
         lk.unlock();
+        sl.scene = ctx_scene(s->ctx);  // kept until the batch completes
+        sl.bounded = false;
         int rc = render_setup(sl.work, sl.views.data(), (int)sl.views.size(), sl.stream);
         if (rc == NES_OK) rc = render_prepare_capacity(sl.work, s->cfg.precision);
         if (rc == NES_OK) rc = launch_batch(s, sl);
@@ -308,7 +386,7 @@ static void scheduler_loop(nes_server *s) {
       }
       bool busy = false;
       for (auto &slp : s->slots) busy = busy || slp->stage != 0;
-      if (s->stop && !busy && s->waiting.empty()) break;
+      if (s->stop && !busy && (s->waiting.empty() || s->hold)) break;
       if (!progressed) {
         if (busy)
           s->cv_work.wait_for(lk, std::chrono::microseconds(50));
@@ -317,6 +395,96 @@ static void scheduler_loop(nes_server *s) {
       }
     }
   }
+}
+
+static bool planes_ok(const nes_frame_planes &p, const nes_camera &c) {
+  return p.data[0] && p.data[1] && p.data[2] && p.linesize[0] >= (int32_t)c.width &&
+         p.linesize[1] >= (int32_t)(c.width / 2) && p.linesize[2] >= (int32_t)(c.width / 2);
+}
+
+static int submit(nes_server *s, uint32_t client, const nes_camera *views, int n_views,
+                  const nes_frame_planes *planes, const nes_frame_planes *dplanes,
+                  uint64_t *ticket) {
+  if (!s || !views || n_views < 1 || n_views > kMaxViews || !ticket) return NES_EINVAL;
+  const bool depth = s->cfg.depth_stream != 0;
+  auto rq = std::make_shared<Request>();
+  rq->client = client;
+  rq->n_views = n_views;
+  rq->caller = planes != nullptr;
+  size_t total = 0;
+  for (int v = 0; v < n_views; ++v) {
+    const nes_camera &c = views[v];
+    if (c.width < 2 || c.height < 2 || (c.width & 1) || (c.height & 1) || c.width > 16384 ||
+        c.height > 16384 || !(c.fov_y_deg > 0.0f && c.fov_y_deg < 180.0f))
+      return NES_EINVAL;
+    if (rq->caller && (!planes_ok(planes[v], c) || (depth && (!dplanes || !planes_ok(dplanes[v], c))))) {
+      set_last_error("caller planes missing or narrower than the frame");
+      return NES_EINVAL;
+    }
+    rq->views[v] = c;
+    total += yuv_bytes(c) * (depth ? 2 : 1);
+  }
+  if (!rq->caller) {
+    NES_TRY(s->pinned.get(total, &rq->yuv));
+    size_t off = 0;
+    for (int v = 0; v < n_views; ++v) {
+      rq->planes[v] = tight_planes(rq->yuv.ptr + off, views[v]);
+      off += yuv_bytes(views[v]);
+      if (depth) {
+        rq->dplanes[v] = tight_planes(rq->yuv.ptr + off, views[v]);
+        off += yuv_bytes(views[v]);
+      }
+    }
+  } else {
+    for (int v = 0; v < n_views; ++v) {
+      rq->planes[v] = planes[v];
+      if (depth) rq->dplanes[v] = dplanes[v];
+    }
+  }
+  rq->arrival = Clock::now();
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stop) {
+      s->pinned.put(rq->yuv);
+      return NES_EINVAL;
+    }
+    if (s->waiting.size() >= s->cfg.queue_capacity) {
+      s->pinned.put(rq->yuv);
+      return NES_EBUSY;
+    }
+    // latest pose wins: a newer request of the same client replaces one that
+    // has not been admitted yet, and is the frame that serves it
+    auto it = s->client_waiting.find(client);
+    if (it != s->client_waiting.end()) {
+      auto old = it->second;
+      old->state = ReqState::kSuperseded;
+      old->status = NES_ECANCELED_;
+      rq->replaced = old->replaced;
+      rq->replaced.push_back(old->arrival);
+      s->waiting.erase(std::remove(s->waiting.begin(), s->waiting.end(), old), s->waiting.end());
+      s->superseded++;
+    }
+    rq->ticket = s->next_ticket++;
+    s->tickets[rq->ticket] = rq;
+    s->client_waiting[client] = rq;
+    s->waiting.push_back(rq);
+    if (!s->any_arrival) {
+      s->any_arrival = true;
+      s->first_arrival = rq->arrival;
+    }
+    *ticket = rq->ticket;
+  }
+  s->cv_work.notify_one();
+  s->cv_done.notify_all();
+  return NES_OK;
+}
+
+static double percentile(std::vector<double> l, double p) {
+  if (l.empty()) return 0.0;
+  std::sort(l.begin(), l.end());
+  const double idx = p * (double)(l.size() - 1);
+  const size_t lo = (size_t)std::floor(idx), hi = std::min(lo + 1, l.size() - 1);
+  return l[lo] + (l[hi] - l[lo]) * (idx - (double)lo);
 }
 
 }  // namespace nes
@@ -333,20 +501,30 @@ int nes_server_create(nes_ctx *ctx, const nes_server_config *cfg, nes_server **o
   if (!s->cfg.max_batch_rays) s->cfg.max_batch_rays = 1u << 24;
   if (!s->cfg.max_inflight) s->cfg.max_inflight = 3;
   if (!s->cfg.queue_capacity) s->cfg.queue_capacity = 256;
+  if (!(s->cfg.depth_far > 0.0f)) s->cfg.depth_far = NES_DEPTH_FAR_DEFAULT;
   if (s->cfg.max_batch_rays > kMaxSlots) return NES_ERANGE;
-  if (s->cfg.precision != NES_PRECISION_FP32 && s->cfg.precision != NES_PRECISION_MIXED) return NES_EINVAL;
+  if (s->cfg.precision != NES_PRECISION_FP32 && s->cfg.precision != NES_PRECISION_MIXED &&
+      s->cfg.precision != NES_PRECISION_FP32_VALU)
+    return NES_EINVAL;
   if (s->cfg.sws_mode != NES_SWS_X86 && s->cfg.sws_mode != NES_SWS_C) return NES_EINVAL;
   if (s->cfg.overlay_flags & ~NES_OVERLAY_ALL) return NES_EINVAL;
-  // admission by memory: as many batch slots as the budget holds
-  const size_t per_slot = slot_bytes(s->cfg.max_batch_rays, s->cfg.precision);
+  if (s->cfg.depth_stream > 1) return NES_EINVAL;
+  // admission by memory: as many batch slots as the budget holds, each
+  // rendering within an equal share of it
+  const bool depth = s->cfg.depth_stream != 0;
+  const size_t per_slot = slot_bytes(s->cfg.max_batch_rays, s->cfg.precision, depth);
   const uint64_t afford = s->cfg.hbm_budget_bytes / per_slot;
   if (afford < 1) {
     set_last_error("HBM budget below one batch slot");
     return NES_ENOMEM;
   }
   const uint32_t n_slots = (uint32_t)std::min<uint64_t>(s->cfg.max_inflight, afford);
+  const uint64_t share = s->cfg.hbm_budget_bytes / n_slots;
+  // the YUV staging and overlay buffers of a slot sit outside its render work
+  const uint64_t staging = (uint64_t)s->cfg.max_batch_rays * (depth ? 4 : 2) + (2u << 20);
   for (uint32_t i = 0; i < n_slots; ++i) {
     auto sl = std::make_unique<Slot>();
+    sl->work.budget = share - staging;
     NES_HIP_TRY(hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking));
     NES_HIP_TRY(hipEventCreateWithFlags(&sl->ev_done, hipEventDisableTiming));
     s->slots.push_back(std::move(sl));
@@ -386,54 +564,23 @@ int nes_server_destroy(nes_server *s) {
 
 int nes_server_submit(nes_server *s, uint32_t client, const nes_camera *views, int n_views,
                       uint64_t *ticket) {
-  if (!s || !views || n_views < 1 || n_views > kMaxViews || !ticket) return NES_EINVAL;
-  auto rq = std::make_shared<Request>();
-  rq->client = client;
-  rq->n_views = n_views;
-  size_t total = 0;
-  for (int v = 0; v < n_views; ++v) {
-    const nes_camera &c = views[v];
-    if (c.width < 2 || c.height < 2 || (c.width & 1) || (c.height & 1) || c.width > 16384 ||
-        c.height > 16384 || !(c.fov_y_deg > 0.0f && c.fov_y_deg < 180.0f))
-      return NES_EINVAL;
-    rq->views[v] = c;
-    rq->view_off[v] = total;
-    total += yuv_bytes(c);
-  }
-  NES_TRY(s->pinned.get(total, &rq->yuv));
-  rq->arrival = Clock::now();
+  return submit(s, client, views, n_views, nullptr, nullptr, ticket);
+}
+
+int nes_server_submit_into(nes_server *s, uint32_t client, const nes_camera *views, int n_views,
+                           const nes_frame_planes *planes, const nes_frame_planes *depth_planes,
+                           uint64_t *ticket) {
+  if (!planes) return NES_EINVAL;
+  return submit(s, client, views, n_views, planes, depth_planes, ticket);
+}
+
+int nes_server_hold(nes_server *s, int hold) {
+  if (!s) return NES_EINVAL;
   {
     std::lock_guard<std::mutex> lk(s->mu);
-    if (s->stop) {
-      s->pinned.put(rq->yuv);
-      return NES_EINVAL;
-    }
-    if (s->waiting.size() >= s->cfg.queue_capacity) {
-      s->pinned.put(rq->yuv);
-      return NES_EBUSY;
-    }
-    // latest pose wins: a newer request of the same client replaces one that
-    // has not been admitted yet
-    auto it = s->client_waiting.find(client);
-    if (it != s->client_waiting.end()) {
-      auto old = it->second;
-      old->state = ReqState::kSuperseded;
-      old->status = NES_ECANCELED_;
-      s->waiting.erase(std::remove(s->waiting.begin(), s->waiting.end(), old), s->waiting.end());
-      s->superseded++;
-    }
-    rq->ticket = s->next_ticket++;
-    s->tickets[rq->ticket] = rq;
-    s->client_waiting[client] = rq;
-    s->waiting.push_back(rq);
-    if (!s->any_arrival) {
-      s->any_arrival = true;
-      s->first_arrival = rq->arrival;
-    }
-    *ticket = rq->ticket;
+    s->hold = hold != 0;
   }
-  s->cv_work.notify_one();
-  s->cv_done.notify_all();
+  s->cv_work.notify_all();
   return NES_OK;
 }
 
@@ -454,18 +601,29 @@ int nes_server_wait(nes_server *s, uint64_t ticket, int timeout_ms, nes_frame_re
   out->n_views = (uint32_t)rq->n_views;
   out->status = rq->status;
   if (rq->state == ReqState::kDone) {
+    const bool depth = s->cfg.depth_stream != 0;
     for (int v = 0; v < rq->n_views; ++v) {
       const nes_camera &c = rq->views[v];
       out->width[v] = c.width;
       out->height[v] = c.height;
-      out->y[v] = rq->yuv.ptr + rq->view_off[v];
-      out->u[v] = out->y[v] + (size_t)c.width * c.height;
-      out->v[v] = out->u[v] + (size_t)(c.width / 2) * (c.height / 2);
+      out->y[v] = rq->planes[v].data[0];
+      out->u[v] = rq->planes[v].data[1];
+      out->v[v] = rq->planes[v].data[2];
+      for (int p = 0; p < 3; ++p) out->linesize[v][p] = rq->planes[v].linesize[p];
+      if (depth) {
+        out->depth_y[v] = rq->dplanes[v].data[0];
+        out->depth_u[v] = rq->dplanes[v].data[1];
+        out->depth_v[v] = rq->dplanes[v].data[2];
+        for (int p = 0; p < 3; ++p) out->depth_linesize[v][p] = rq->dplanes[v].linesize[p];
+      }
     }
     out->latency_ms = ms_between(rq->arrival, rq->done);
     out->queue_ms = ms_between(rq->arrival, rq->admitted);
     out->rays = rq->rays;
     out->samples = rq->samples;
+    out->batch = rq->batch;
+    out->batch_requests = rq->batch_requests;
+    out->batch_passes = rq->batch_passes;
   }
   return rq->status;
 }
@@ -496,21 +654,18 @@ int nes_server_stats_get(nes_server *s, nes_server_stats *st) {
   st->rays = s->rays;
   st->samples = s->samples;
   st->batches = s->batches;
+  st->composited = s->composited;
+  st->bounded_batches = s->bounded_batches;
   st->slots = (uint32_t)s->slots.size();
-  std::vector<double> l = s->latencies_ms;
-  if (!l.empty()) {
-    std::sort(l.begin(), l.end());
-    auto pct = [&](double p) {
-      const double idx = p * (double)(l.size() - 1);
-      const size_t lo = (size_t)std::floor(idx), hi = std::min(lo + 1, l.size() - 1);
-      return l[lo] + (l[hi] - l[lo]) * (idx - (double)lo);
-    };
-    st->p50_ms = pct(0.50);
-    st->p99_ms = pct(0.99);
-    st->max_ms = l.back();
+  if (!s->latencies_ms.empty()) {
+    st->p50_ms = percentile(s->latencies_ms, 0.50);
+    st->p99_ms = percentile(s->latencies_ms, 0.99);
+    st->max_ms = *std::max_element(s->latencies_ms.begin(), s->latencies_ms.end());
     double sum = 0;
-    for (double x : l) sum += x;
-    st->mean_ms = sum / (double)l.size();
+    for (double x : s->latencies_ms) sum += x;
+    st->mean_ms = sum / (double)s->latencies_ms.size();
+    st->p50_all_ms = percentile(s->latencies_all_ms, 0.50);
+    st->p99_all_ms = percentile(s->latencies_all_ms, 0.99);
   }
   if (s->any_arrival && s->completed) {
     st->wall_s = std::chrono::duration<double>(s->last_done - s->first_arrival).count();
@@ -523,7 +678,9 @@ int nes_server_reset_stats(nes_server *s) {
   if (!s) return NES_EINVAL;
   std::lock_guard<std::mutex> lk(s->mu);
   s->latencies_ms.clear();
+  s->latencies_all_ms.clear();
   s->completed = s->superseded = s->failed = s->rays = s->samples = s->batches = 0;
+  s->composited = s->bounded_batches = 0;
   s->any_arrival = false;
   return NES_OK;
 }

@@ -418,7 +418,7 @@ __global__ __launch_bounds__(256) void train_dcomposite(
   uint32_t cnt = 0, base = 0, used = 0;
   float g[3] = {0.0f, 0.0f, 0.0f};
   if (valid) {
-    cnt = counts[slot];
+    cnt = counts[slot] & kCountMask;
     base = offsets[slot];
     if (cnt) {
       const RayAcc a = acc[slot];
@@ -589,7 +589,6 @@ struct nes_trainer {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   uint64_t step = 0;
-  uint32_t last_rays = 0;
   std::mutex mu;
 };
 
@@ -852,10 +851,10 @@ int nes_trainer_set_params(nes_trainer *t, const float *table, const float *weig
   return NES_OK;
 }
 
-int nes_trainer_forward(nes_trainer *t, const float *ray_o, const float *ray_d, uint32_t n,
-                        float *rgb) {
-  if (!t || !ray_o || !ray_d || !n || n > kMaxSlots) return NES_EINVAL;
-  std::lock_guard<std::mutex> lk(t->mu);
+// Forward of explicit rays (n x 3 origins, unit directions) into the work
+// area, leaving their per-ray colour in its outputs.
+static int trainer_forward_rays(nes_trainer *t, const float *ray_o, const float *ray_d,
+                                uint32_t n) {
   hipStream_t s = t->stream;
   NES_TRY(trainer_prepare(t, n));
   NES_TRY(t->xo.reserve(12ull * n));
@@ -868,8 +867,15 @@ int nes_trainer_forward(nes_trainer *t, const float *ray_o, const float *ray_d, 
   ra.acc = t->work.acc.as<RayAcc>();
   ra.ray_o = t->xo.as<float>();
   ra.ray_d = t->xd.as<float>();
-  NES_TRY(trainer_forward(t, n, ra));
-  t->last_rays = n;
+  return trainer_forward(t, n, ra);
+}
+
+int nes_trainer_forward(nes_trainer *t, const float *ray_o, const float *ray_d, uint32_t n,
+                        float *rgb) {
+  if (!t || !ray_o || !ray_d || !n || n > kMaxSlots) return NES_EINVAL;
+  std::lock_guard<std::mutex> lk(t->mu);
+  hipStream_t s = t->stream;
+  NES_TRY(trainer_forward_rays(t, ray_o, ray_d, n));
   if (rgb) {
     const OutLayout L = out_layout(t->work.out_px);
     NES_HIP_TRY(hipMemcpyAsync(rgb, t->work.out.as<uint8_t>() + L.rgbf, 12ull * n,
@@ -879,12 +885,16 @@ int nes_trainer_forward(nes_trainer *t, const float *ray_o, const float *ray_d, 
   return NES_OK;
 }
 
-int nes_trainer_backward(nes_trainer *t, const float *d_rgb, float *grad_table,
-                         float *grad_weights) {
-  if (!t || !d_rgb || !t->last_rays) return NES_EINVAL;
+// The backward takes the rays again and recomputes their forward (samples,
+// features, MLP outputs, compositing state) before differentiating, so it
+// depends on no state left by an earlier call: any number of forwards,
+// steps or batch gradients may come between a forward and its backward.
+int nes_trainer_backward(nes_trainer *t, const float *ray_o, const float *ray_d, uint32_t n,
+                         const float *d_rgb, float *grad_table, float *grad_weights) {
+  if (!t || !ray_o || !ray_d || !d_rgb || !n || n > kMaxSlots) return NES_EINVAL;
   std::lock_guard<std::mutex> lk(t->mu);
   hipStream_t s = t->stream;
-  const uint32_t n = t->last_rays;
+  NES_TRY(trainer_forward_rays(t, ray_o, ray_d, n));
   NES_TRY(t->dCb.reserve(12ull * n));
   t->dC = t->dCb.as<float>();
   NES_HIP_TRY(hipMemcpyAsync(t->dC, d_rgb, 12ull * n, hipMemcpyHostToDevice, s));
@@ -897,6 +907,84 @@ int nes_trainer_backward(nes_trainer *t, const float *d_rgb, float *grad_table,
                                hipMemcpyDeviceToHost, s));
   zero_f32<<<1024, 256, 0, s>>>(t->grads, t->n_params);
   NES_HIP_TRY(hipStreamSynchronize(s));
+  return NES_OK;
+}
+
+// ------------------------------------------------------------ checkpoint
+// File layout (little-endian): "NESCKPT1", u32 version 1, u32 weights,
+// u64 table floats, u64 step, then f32 parameters (table, weights), first
+// moments and second moments, n_params each.
+namespace {
+const char kCkptMagic[8] = {'N', 'E', 'S', 'C', 'K', 'P', 'T', '1'};
+struct CkptHeader {
+  char magic[8];
+  uint32_t version, weights;
+  uint64_t table, step;
+};
+}  // namespace
+
+int nes_trainer_save(nes_trainer *t, const char *path) {
+  if (!t || !path) return NES_EINVAL;
+  std::lock_guard<std::mutex> lk(t->mu);
+  NES_HIP_TRY(hipStreamSynchronize(t->stream));
+  std::vector<float> buf(t->n_params);
+  FILE *f = std::fopen(path, "wb");
+  if (!f) {
+    set_last_error(std::string("cannot create ") + path);
+    return NES_EIO;
+  }
+  CkptHeader h;
+  std::memcpy(h.magic, kCkptMagic, 8);
+  h.version = 1;
+  h.weights = kWeights;
+  h.table = t->n_table;
+  h.step = t->step;
+  bool ok = std::fwrite(&h, sizeof(h), 1, f) == 1;
+  for (float *src : {t->params, t->m, t->v}) {
+    if (!ok) break;
+    if (hipMemcpy(buf.data(), src, t->n_params * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) {
+      ok = false;
+      break;
+    }
+    ok = std::fwrite(buf.data(), sizeof(float), t->n_params, f) == t->n_params;
+  }
+  ok = (std::fclose(f) == 0) && ok;
+  if (!ok) {
+    set_last_error(std::string("writing checkpoint ") + path + " failed");
+    return NES_EIO;
+  }
+  return NES_OK;
+}
+
+int nes_trainer_load(nes_trainer *t, const char *path) {
+  if (!t || !path) return NES_EINVAL;
+  std::lock_guard<std::mutex> lk(t->mu);
+  NES_HIP_TRY(hipStreamSynchronize(t->stream));
+  FILE *f = std::fopen(path, "rb");
+  if (!f) {
+    set_last_error(std::string("cannot open ") + path);
+    return NES_EIO;
+  }
+  CkptHeader h;
+  if (std::fread(&h, sizeof(h), 1, f) != 1 || std::memcmp(h.magic, kCkptMagic, 8) != 0 ||
+      h.version != 1 || h.weights != (uint32_t)kWeights || h.table != t->n_table) {
+    std::fclose(f);
+    set_last_error("not a checkpoint of this scene representation");
+    return NES_EINVAL;
+  }
+  // read everything before touching the trainer, so a short file changes nothing
+  std::vector<float> buf(3 * t->n_params);
+  const bool ok = std::fread(buf.data(), sizeof(float), buf.size(), f) == buf.size();
+  std::fclose(f);
+  if (!ok) {
+    set_last_error("checkpoint file is truncated");
+    return NES_EINVAL;
+  }
+  float *dst[3] = {t->params, t->m, t->v};
+  for (int i = 0; i < 3; ++i)
+    NES_HIP_TRY(hipMemcpy(dst[i], buf.data() + (size_t)i * t->n_params, t->n_params * sizeof(float),
+                          hipMemcpyHostToDevice));
+  t->step = h.step;
   return NES_OK;
 }
 

@@ -16,9 +16,10 @@
 //   FrameRequest  1: index  2: camera (message)  3: is_left
 //   RenderedFrame 1: index  2: camera  3: is_left  6: frame (RGB24 bytes)
 //                 7: depth (GRAY8 bytes)
-// Unknown fields are skipped.  GRAY8 depth = clamp(floor(255 d / far + 0.5))
-// with a configurable far distance (the reference's renderer quantises depth
-// itself; its rule is not in the reference).
+// Unknown fields are skipped.  GRAY8 depth = clamp(floor(fma(d, 255 / far,
+// 0.5))) in FP32, computed on the device by the render (nes_view_out
+// depth_gray8), with a configurable far distance (the reference's renderer
+// quantises depth itself; its rule is not in the reference).
 #include <arpa/inet.h>
 #include <netinet/in.h>
 #include <sys/socket.h>
@@ -275,7 +276,6 @@ bool write_full(int fd, const void *buf, size_t n) {
 
 void serve_connection(nes_wire_server *s, int fd) {
   std::vector<uint8_t> msg, rgb, gray, out;
-  std::vector<float> depth;
   while (!s->stop) {
     uint64_t len = 0;  // host-endian size_t, as src/server.cpp:50
     if (!read_full(fd, &len, sizeof(len))) break;
@@ -287,14 +287,10 @@ void serve_connection(nes_wire_server *s, int fd) {
     if (cam.width < 1 || cam.height < 1 || cam.width > 16384 || cam.height > 16384) break;
     const size_t px = (size_t)cam.width * cam.height;
     rgb.resize(3 * px);
-    depth.resize(px);
     gray.resize(px);
-    nes_view_out vo{rgb.data(), depth.data(), nullptr, nullptr};
+    // RGB24 and the GRAY8 depth plane both come from the device
+    nes_view_out vo{rgb.data(), nullptr, nullptr, nullptr, gray.data(), s->depth_far};
     if (nes_render(s->ctx, s->precision, &cam, 1, &vo, 0, nullptr) != NES_OK) break;
-    for (size_t i = 0; i < px; ++i) {
-      const float q = std::floor(depth[i] / s->depth_far * 255.0f + 0.5f);
-      gray[i] = (uint8_t)std::fmin(std::fmax(q, 0.0f), 255.0f);
-    }
     size_t n = 0;
     nes_wire_encode_frame(&cam, rgb.data(), gray.data(), nullptr, 0, &n);
     out.resize(n);

@@ -25,6 +25,8 @@ NES_OK = 0
 NES_EAGAIN = -11
 PRECISION_FP32 = 0
 PRECISION_MIXED = 1
+PRECISION_FP32_VALU = 2
+DEPTH_FAR_DEFAULT = 4.0
 SWS_X86 = 0
 SWS_C = 1
 TABLE_ENTRIES = 6098925
@@ -51,20 +53,28 @@ class SceneDesc(C.Structure):
 
 class ViewOut(C.Structure):
     _fields_ = [("rgb24", C.POINTER(C.c_uint8)), ("depth", C.POINTER(C.c_float)),
-                ("rgb_f32", C.POINTER(C.c_float)), ("n_samples", C.POINTER(C.c_uint16))]
+                ("rgb_f32", C.POINTER(C.c_float)), ("n_samples", C.POINTER(C.c_uint16)),
+                ("depth_gray8", C.POINTER(C.c_uint8)), ("depth_far", C.c_float)]
 
 
 class RenderStats(C.Structure):
     _fields_ = [("rays", C.c_uint64), ("samples", C.c_uint64), ("ms_total", C.c_double),
                 ("ms_eval", C.c_double), ("ms_march", C.c_double), ("ms_composite", C.c_double),
-                ("ms_encode", C.c_double), ("ms_mlp", C.c_double)]
+                ("ms_encode", C.c_double), ("ms_mlp", C.c_double), ("composited", C.c_uint64),
+                ("terminated_rays", C.c_uint64), ("passes", C.c_uint32)]
 
 
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_batch_rays", C.c_uint32),
                 ("max_inflight", C.c_uint32), ("queue_capacity", C.c_uint32),
                 ("precision", C.c_int), ("sws_mode", C.c_int), ("font", C.c_void_p),
-                ("overlay_flags", C.c_uint32)]
+                ("overlay_flags", C.c_uint32), ("depth_stream", C.c_uint32),
+                ("depth_far", C.c_float)]
+
+
+class FramePlanes(C.Structure):
+    """nes_frame_planes: an encoder frame's data[0..2] / linesize[0..2]."""
+    _fields_ = [("data", C.POINTER(C.c_uint8) * 3), ("linesize", C.c_int32 * 3)]
 
 
 class Glyph(C.Structure):
@@ -82,7 +92,11 @@ class FrameResult(C.Structure):
                 ("y", C.POINTER(C.c_uint8) * 2), ("u", C.POINTER(C.c_uint8) * 2),
                 ("v", C.POINTER(C.c_uint8) * 2), ("latency_ms", C.c_double),
                 ("queue_ms", C.c_double), ("rays", C.c_uint64), ("samples", C.c_uint64),
-                ("status", C.c_int)]
+                ("status", C.c_int), ("linesize", (C.c_int32 * 3) * 2),
+                ("depth_y", C.POINTER(C.c_uint8) * 2), ("depth_u", C.POINTER(C.c_uint8) * 2),
+                ("depth_v", C.POINTER(C.c_uint8) * 2), ("depth_linesize", (C.c_int32 * 3) * 2),
+                ("batch", C.c_uint64), ("batch_requests", C.c_uint32),
+                ("batch_passes", C.c_uint32)]
 
 
 class ServerStats(C.Structure):
@@ -90,7 +104,9 @@ class ServerStats(C.Structure):
                 ("rays", C.c_uint64), ("samples", C.c_uint64), ("batches", C.c_uint64),
                 ("slots", C.c_uint32), ("p50_ms", C.c_double), ("p99_ms", C.c_double),
                 ("mean_ms", C.c_double), ("max_ms", C.c_double), ("wall_s", C.c_double),
-                ("mrays_per_s", C.c_double)]
+                ("mrays_per_s", C.c_double), ("p50_all_ms", C.c_double),
+                ("p99_all_ms", C.c_double), ("composited", C.c_uint64),
+                ("bounded_batches", C.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -155,7 +171,10 @@ _sig("nes_trainer_get_params", C.c_int, C.c_void_p, _P(C.c_float), _P(C.c_float)
 _sig("nes_trainer_set_params", C.c_int, C.c_void_p, _P(C.c_float), _P(C.c_float))
 _sig("nes_trainer_forward", C.c_int, C.c_void_p, _P(C.c_float), _P(C.c_float), C.c_uint32,
      _P(C.c_float))
-_sig("nes_trainer_backward", C.c_int, C.c_void_p, _P(C.c_float), _P(C.c_float), _P(C.c_float))
+_sig("nes_trainer_backward", C.c_int, C.c_void_p, _P(C.c_float), _P(C.c_float), C.c_uint32,
+     _P(C.c_float), _P(C.c_float), _P(C.c_float))
+_sig("nes_trainer_save", C.c_int, C.c_void_p, C.c_char_p)
+_sig("nes_trainer_load", C.c_int, C.c_void_p, C.c_char_p)
 _sig("nes_wire_decode_request", C.c_int, C.c_char_p, C.c_size_t, C.c_float, _P(Camera))
 _sig("nes_wire_encode_frame", C.c_int, _P(Camera), _P(C.c_uint8), _P(C.c_uint8), C.c_char_p,
      C.c_size_t, _P(C.c_size_t))
@@ -175,6 +194,9 @@ _sig("nes_frameprep_overlay", C.c_int, C.c_void_p, _P(Camera), C.c_char_p, C.c_u
 _sig("nes_server_create", C.c_int, C.c_void_p, _P(ServerConfig), _P(C.c_void_p))
 _sig("nes_server_destroy", C.c_int, C.c_void_p)
 _sig("nes_server_submit", C.c_int, C.c_void_p, C.c_uint32, _P(Camera), C.c_int, _P(C.c_uint64))
+_sig("nes_server_submit_into", C.c_int, C.c_void_p, C.c_uint32, _P(Camera), C.c_int,
+     _P(FramePlanes), _P(FramePlanes), _P(C.c_uint64))
+_sig("nes_server_hold", C.c_int, C.c_void_p, C.c_int)
 _sig("nes_server_wait", C.c_int, C.c_void_p, C.c_uint64, C.c_int, _P(FrameResult))
 _sig("nes_server_release", C.c_int, C.c_void_p, C.c_uint64)
 _sig("nes_server_stats_get", C.c_int, C.c_void_p, _P(ServerStats))
@@ -182,6 +204,8 @@ _sig("nes_server_reset_stats", C.c_int, C.c_void_p)
 _sig("nes_server_simulate", C.c_int, C.c_void_p, _P(SimConfig), _P(ServerStats))
 _sig("nes_device_alloc", C.c_int, C.c_size_t, _P(C.c_void_p))
 _sig("nes_device_free", C.c_int, C.c_void_p)
+_sig("nes_host_alloc", C.c_int, C.c_size_t, _P(C.c_void_p))
+_sig("nes_host_free", C.c_int, C.c_void_p)
 _sig("nes_memcpy_h2d", C.c_int, C.c_void_p, C.c_void_p, C.c_size_t)
 _sig("nes_memcpy_d2h", C.c_int, C.c_void_p, C.c_void_p, C.c_size_t)
 _sig("nes_device_copy", C.c_int, C.c_void_p, C.c_void_p, C.c_size_t, _P(C.c_float))
@@ -267,8 +291,10 @@ class Context:
         check(lib().nes_ctx_update_scene(self.handle, C.byref(desc)), "nes_ctx_update_scene")
         self._keep = keep
 
-    def render(self, cams, precision=PRECISION_MIXED, steps_per_pass=0):
-        """Render views; returns ([dict per view], RenderStats)."""
+    def render(self, cams, precision=PRECISION_MIXED, steps_per_pass=0, depth_far=0.0):
+        """Render views; returns ([dict per view], RenderStats).  Each dict
+        holds rgb24, depth, rgb (float), n_samples and depth_gray8 (the
+        RenderedFrame depth plane, quantised against depth_far)."""
         if isinstance(cams, Camera):
             cams = [cams]
         n = len(cams)
@@ -279,9 +305,11 @@ class Context:
             r = {"rgb24": np.empty((c.height, c.width, 3), np.uint8),
                  "depth": np.empty((c.height, c.width), np.float32),
                  "rgb": np.empty((c.height, c.width, 3), np.float32),
-                 "n_samples": np.empty((c.height, c.width), np.uint16)}
+                 "n_samples": np.empty((c.height, c.width), np.uint16),
+                 "depth_gray8": np.empty((c.height, c.width), np.uint8)}
             outs[i] = ViewOut(_ptr(r["rgb24"], C.c_uint8), _ptr(r["depth"], C.c_float),
-                              _ptr(r["rgb"], C.c_float), _ptr(r["n_samples"], C.c_uint16))
+                              _ptr(r["rgb"], C.c_float), _ptr(r["n_samples"], C.c_uint16),
+                              _ptr(r["depth_gray8"], C.c_uint8), depth_far)
             results.append(r)
         st = RenderStats()
         check(lib().nes_render(self.handle, precision, arr, n, outs, steps_per_pass, C.byref(st)),
@@ -316,6 +344,29 @@ class Context:
 
 
 # ------------------------------------------------------------ frame prep
+
+def frameprep_pitched(rgb_rows_a, w, depth_a=None, rgb_rows_b=None, depth_b=None, mode=SWS_X86,
+                      y_stride=None, uv_stride=None):
+    """Nearer-depth composite + RGB24 -> YUV420P with row pitches, like
+    sws_scale on planes with linesizes.  rgb_rows_*: (h, rgb_stride) uint8,
+    pixels in the first 3w bytes of each row.  Returns the whole padded
+    (h, y_stride) and (h/2, uv_stride) output arrays, pre-filled with 0xA5."""
+    h, rgb_stride = rgb_rows_a.shape
+    y_stride = y_stride or w
+    uv_stride = uv_stride or w // 2
+    ra = np.ascontiguousarray(rgb_rows_a, np.uint8)
+    rb = None if rgb_rows_b is None else np.ascontiguousarray(rgb_rows_b, np.uint8)
+    da = None if depth_a is None else np.ascontiguousarray(depth_a, np.float32)
+    db = None if depth_b is None else np.ascontiguousarray(depth_b, np.float32)
+    y = np.full((h, y_stride), 0xA5, np.uint8)
+    u = np.full((h // 2, uv_stride), 0xA5, np.uint8)
+    v = np.full((h // 2, uv_stride), 0xA5, np.uint8)
+    check(lib().nes_frameprep(_ptr(ra, C.c_uint8), _ptr(da, C.c_float), _ptr(rb, C.c_uint8),
+                              _ptr(db, C.c_float), w, h, rgb_stride, _ptr(y, C.c_uint8), y_stride,
+                              _ptr(u, C.c_uint8), uv_stride, _ptr(v, C.c_uint8), uv_stride, mode),
+          "nes_frameprep")
+    return y, u, v
+
 
 def frameprep(rgb_a, depth_a=None, rgb_b=None, depth_b=None, mode=SWS_X86):
     """Nearer-depth composite + RGB24 -> YUV420P (host arrays); returns (y, u, v)."""
@@ -459,12 +510,15 @@ class DeviceBuffer:
             self.ptr = None
 
 
-def frameprep_device(rgb_a, depth_a, rgb_b, depth_b, w, h, n_frames, y, u, v, mode=SWS_X86):
-    """Device-pointer frame prep over n_frames packed frames; returns kernel ms."""
+def frameprep_device(rgb_a, depth_a, rgb_b, depth_b, w, h, n_frames, y, u, v, mode=SWS_X86,
+                     y_stride=None, uv_stride=None):
+    """Device-pointer frame prep over n_frames packed frames (output planes
+    y_stride * h and uv_stride * h/2 bytes apart); returns kernel ms."""
     ms = C.c_float()
     vp = lambda b: b.ptr if b is not None else None
+    ys, uvs = y_stride or w, uv_stride or w // 2
     check(lib().nes_frameprep_device(vp(rgb_a), vp(depth_a), vp(rgb_b), vp(depth_b), w, h,
-                                     n_frames, vp(y), w, vp(u), w // 2, vp(v), w // 2, mode,
+                                     n_frames, vp(y), ys, vp(u), uvs, vp(v), uvs, mode,
                                      C.byref(ms)), "nes_frameprep_device")
     return ms.value
 
@@ -489,15 +543,68 @@ def device_info():
 NES_ECANCELED = -125
 
 
+class HostPlanes:
+    """YUV420P planes of one w x h frame in pinned host memory
+    (nes_host_alloc) with encoder linesizes: FFALIGN(width, align) for Y and
+    FFALIGN(width / 2, align) for U and V, as av_image_alloc(..., align)
+    lays them out (src/base/video/type_managers.cc:116-141)."""
+
+    def __init__(self, w, h, align=32):
+        a = lambda x: (x + align - 1) // align * align
+        self.w, self.h = w, h
+        self.linesize = (a(w), a(w // 2), a(w // 2))
+        self.rows = (h, h // 2, h // 2)
+        sizes = [ls * r for ls, r in zip(self.linesize, self.rows)]
+        p = C.c_void_p()
+        check(lib().nes_host_alloc(sum(sizes), C.byref(p)), "nes_host_alloc")
+        self.ptr, self.nbytes = p, sum(sizes)
+        base = C.cast(p, C.POINTER(C.c_uint8))
+        raw = np.ctypeslib.as_array(base, (self.nbytes,))
+        raw[:] = 0xA5  # padding bytes must survive the server's writes
+        self.raw = raw
+        offs = np.cumsum([0] + sizes[:-1])
+        self.planes = FramePlanes()
+        for i in range(3):
+            self.planes.data[i] = C.cast(C.c_void_p(p.value + int(offs[i])), C.POINTER(C.c_uint8))
+            self.planes.linesize[i] = self.linesize[i]
+        self.views = [raw[o:o + s].reshape(r, ls) for o, s, r, ls in
+                      zip(offs, sizes, self.rows, self.linesize)]
+
+    def yuv(self):
+        """(y, u, v) of the frame (copies, without the row padding)."""
+        return (self.views[0][:, :self.w].copy(), self.views[1][:, :self.w // 2].copy(),
+                self.views[2][:, :self.w // 2].copy())
+
+    def padding_intact(self):
+        return all((v[:, n:] == 0xA5).all() for v, n in
+                   zip(self.views, (self.w, self.w // 2, self.w // 2)))
+
+    def free(self):
+        if self.ptr:
+            check(lib().nes_host_free(self.ptr), "nes_host_free")
+            self.ptr = None
+
+
+def _copy_planes(ptrs, lines, w, h):
+    out = []
+    for p, ls, (pw, ph) in zip(ptrs, lines, ((w, h), (w // 2, h // 2), (w // 2, h // 2))):
+        a = np.ctypeslib.as_array(p, (ph, ls))
+        out.append(a[:, :pw].copy())
+    return tuple(out)
+
+
 class Server:
     """Continuous-batching render server over a Context (nes_server)."""
 
     def __init__(self, ctx, hbm_budget=0, max_batch_rays=0, max_inflight=0, queue_capacity=0,
-                 precision=PRECISION_MIXED, sws_mode=SWS_X86, font=None, overlay_flags=0):
+                 precision=PRECISION_MIXED, sws_mode=SWS_X86, font=None, overlay_flags=0,
+                 depth_stream=False, depth_far=0.0):
         self.ctx = ctx
         self.font = font  # must outlive the server
+        self.depth_stream = bool(depth_stream)
         cfg = ServerConfig(hbm_budget, max_batch_rays, max_inflight, queue_capacity, precision,
-                           sws_mode, font.handle if font is not None else None, overlay_flags)
+                           sws_mode, font.handle if font is not None else None, overlay_flags,
+                           int(depth_stream), depth_far)
         h = C.c_void_p()
         check(lib().nes_server_create(ctx.handle, C.byref(cfg), C.byref(h)), "nes_server_create")
         self.handle = h
@@ -511,22 +618,41 @@ class Server:
               "nes_server_submit")
         return t.value
 
-    def wait(self, ticket, timeout_ms=-1, release=True):
-        """Returns (status, [(y, u, v) per view] copied out, FrameResult)."""
+    def submit_into(self, client, views, planes, depth_planes=None):
+        """Like submit, the planes written into caller-owned HostPlanes."""
+        if isinstance(views, Camera):
+            views = [views]
+        arr = (Camera * len(views))(*views)
+        pa = (FramePlanes * len(views))(*[p.planes for p in planes])
+        da = None if depth_planes is None else (FramePlanes * len(views))(
+            *[p.planes for p in depth_planes])
+        t = C.c_uint64()
+        check(lib().nes_server_submit_into(self.handle, client, arr, len(views), pa, da,
+                                           C.byref(t)), "nes_server_submit_into")
+        return t.value
+
+    def hold(self, on=True):
+        check(lib().nes_server_hold(self.handle, int(on)), "nes_server_hold")
+
+    def wait(self, ticket, timeout_ms=-1, release=True, depth=False):
+        """Returns (status, [(y, u, v) per view] copied out, FrameResult); with
+        depth=True also the depth stream: (status, scene, depth, result)."""
         r = FrameResult()
         rc = lib().nes_server_wait(self.handle, ticket, timeout_ms, C.byref(r))
-        planes = []
+        planes, dplanes = [], []
         if rc == NES_OK:
             for i in range(r.n_views):
                 w, h = r.width[i], r.height[i]
-                y = np.ctypeslib.as_array(r.y[i], (h, w)).copy()
-                u = np.ctypeslib.as_array(r.u[i], (h // 2, w // 2)).copy()
-                v = np.ctypeslib.as_array(r.v[i], (h // 2, w // 2)).copy()
-                planes.append((y, u, v))
+                planes.append(_copy_planes((r.y[i], r.u[i], r.v[i]), r.linesize[i], w, h))
+                if self.depth_stream:
+                    dplanes.append(_copy_planes((r.depth_y[i], r.depth_u[i], r.depth_v[i]),
+                                                r.depth_linesize[i], w, h))
         elif rc not in (NES_ECANCELED, NES_EAGAIN):
             check(rc, "nes_server_wait")
         if release and rc != NES_EAGAIN:
             check(lib().nes_server_release(self.handle, ticket), "nes_server_release")
+        if depth:
+            return rc, planes, dplanes, r
         return rc, planes, r
 
     def stats(self):
@@ -613,13 +739,26 @@ class Trainer:
                                         _ptr(rgb, C.c_float)), "nes_trainer_forward")
         return rgb
 
-    def backward(self, d_rgb):
+    def backward(self, ray_o, ray_d, d_rgb):
+        """Gradients of sum(d_rgb * rgb(ray_o, ray_d)) (the forward of these
+        rays is recomputed inside)."""
+        o = np.ascontiguousarray(ray_o, np.float32)
+        d = np.ascontiguousarray(ray_d, np.float32)
         g = np.ascontiguousarray(d_rgb, np.float32)
+        if g.shape != (len(o), 3) or d.shape != o.shape:
+            raise ValueError("ray_o, ray_d and d_rgb must be n x 3")
         gt = np.empty(TABLE_ENTRIES * 2, np.float32)
         gw = np.empty(MLP_WEIGHTS, np.float32)
-        check(lib().nes_trainer_backward(self.handle, _ptr(g, C.c_float), _ptr(gt, C.c_float),
+        check(lib().nes_trainer_backward(self.handle, _ptr(o, C.c_float), _ptr(d, C.c_float), len(o),
+                                         _ptr(g, C.c_float), _ptr(gt, C.c_float),
                                          _ptr(gw, C.c_float)), "nes_trainer_backward")
         return gt, gw
+
+    def save(self, path):
+        check(lib().nes_trainer_save(self.handle, os.fsencode(path)), "nes_trainer_save")
+
+    def load(self, path):
+        check(lib().nes_trainer_load(self.handle, os.fsencode(path)), "nes_trainer_load")
 
     def close(self):
         if self.handle:

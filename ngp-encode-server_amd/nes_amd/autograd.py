@@ -14,14 +14,22 @@ class RenderRays(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, table, weights, trainer, ray_o, ray_d):
-        trainer.set_params(table.detach().cpu().numpy(), weights.detach().cpu().numpy())
-        rgb = trainer.forward(np.asarray(ray_o, np.float32), np.asarray(ray_d, np.float32))
-        ctx.trainer = trainer
+        t = table.detach().cpu().numpy()
+        w = weights.detach().cpu().numpy()
+        o = np.ascontiguousarray(ray_o, np.float32)
+        d = np.ascontiguousarray(ray_d, np.float32)
+        trainer.set_params(t, w)
+        rgb = trainer.forward(o, d)
+        # everything the backward needs travels with this node: the trainer
+        # is shared by every call, so no per-call state may live in it
+        ctx.trainer, ctx.params, ctx.rays = trainer, (t, w), (o, d)
         return torch.from_numpy(rgb)
 
     @staticmethod
     def backward(ctx, grad_rgb):
-        gt, gw = ctx.trainer.backward(grad_rgb.detach().cpu().numpy().astype(np.float32))
+        ctx.trainer.set_params(*ctx.params)
+        gt, gw = ctx.trainer.backward(*ctx.rays,
+                                      grad_rgb.detach().cpu().numpy().astype(np.float32))
         return torch.from_numpy(gt), torch.from_numpy(gw), None, None, None
 
 

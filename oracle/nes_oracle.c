@@ -490,6 +490,16 @@ void orc_render_pixels(const float *table, const float *weights,
   (void)threads;
 }
 
+void orc_depth_gray8(const float *depth, int n, float far, uint8_t *out) {
+  const float scale = 255.0f / far;
+  for (int i = 0; i < n; ++i) {
+    float q = floorf(fmaf(depth[i], scale, 0.5f));
+    if (!(q >= 0.0f)) q = 0.0f; /* negative or NaN */
+    if (q > 255.0f) q = 255.0f;
+    out[i] = (uint8_t)q;
+  }
+}
+
 uint8_t orc_quantize(double c) {
   double v = floor(c * 255.0 + 0.5);
   if (v < 0.0) v = 0.0;

@@ -139,6 +139,11 @@ int orc_sws_vchroma_filter(int h, int16_t *coef, int32_t *pos, int max_size);
 int orc_rgb24_to_yuv420p(const uint8_t *rgb, int rgb_stride, int w, int h,
                          uint8_t *y, int y_stride, uint8_t *u, int u_stride,
                          uint8_t *v, int v_stride, int mode);
+/* GRAY8 depth plane of a RenderedFrame from FP32 expected depth:
+ * g = clamp(floor(fma(d, 255 / far, 0.5)), 0, 255) in FP32, NaN -> 0.  The
+ * reference's external renderer quantised depth itself (proto/nes.proto:24
+ * carries the bytes); this is the rule the library defines. */
+void orc_depth_gray8(const float *depth, int n, float far, uint8_t *out);
 /* GRAY8 (full range) -> YUV420P (limited range), the depth stream. */
 int orc_gray8_to_yuv420p(const uint8_t *g, int g_stride, int w, int h,
                          uint8_t *y, int y_stride, uint8_t *u, int u_stride,

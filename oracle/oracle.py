@@ -78,6 +78,7 @@ def _declare(L):
         "orc_render_pixels": (None, [f32p, f32p, u32p, _p(Camera), u32p, C.c_int, f64p, f64p,
                                      i32p, C.c_int]),
         "orc_quantize": (C.c_uint8, [C.c_double]),
+        "orc_depth_gray8": (None, [f32p, C.c_int, C.c_float, u8p]),
         "orc_loss_grad": (C.c_double, [f32p, f32p, u32p, C.c_int, f32p, f32p, f32p, f64p, f64p]),
         "orc_loss_perturbed": (C.c_double, [f32p, f32p, u32p, C.c_int, f32p, f32p, f32p,
                                             C.c_long, C.c_long, C.c_double]),
@@ -274,6 +275,14 @@ def render_string(rgb, glyphs, bitmaps, position, text):
                             bm.ctypes.data_as(C.POINTER(C.c_uint8)), position,
                             text.encode("latin-1"))
     return rgb
+
+
+def depth_gray8(depth, far=4.0):
+    """GRAY8 depth plane of FP32 depth (the library's quantisation rule)."""
+    d = np.ascontiguousarray(depth, np.float32)
+    out = np.empty(d.shape, np.uint8)
+    lib().orc_depth_gray8(ptr(d, C.c_float), d.size, far, ptr(out, C.c_uint8))
+    return out
 
 
 def set_frameprep_threads(n):

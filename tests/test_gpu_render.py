@@ -201,3 +201,56 @@ def test_batch_beyond_slot_limit_is_refused(nes, gpu_ctx):
     assert e.value.code == -34
     (out,), _ = gpu_ctx.render(nes.camera(64, 48, nes.orbit_camera(30.0, 20.0)))
     assert out["rgb24"].shape[:2] == (48, 64)
+
+
+@pytest.mark.parametrize("prec", [FP32, MIXED])
+def test_adaptive_passes_drop_terminated_rays(nes, oracle, prec):
+    """Opaque scene (every ray stops at T < 1e-4 after its first sample): the
+    first frame runs one pass and evaluates every occupied sample; the next
+    frame runs two passes, the first giving each ray the samples the
+    terminated rays needed (one here), so under 10 % of the evaluated samples
+    lie behind termination; pixels are bit-identical in every mode."""
+    scene = _const_scene(1500.0)
+    cam, _ = cams(nes, oracle, 256, 192)
+    with nes.Context(*scene) as ctx:
+        (a,), s1 = ctx.render(cam, precision=prec)
+        (b,), s2 = ctx.render(cam, precision=prec)
+        (c,), s3 = ctx.render(cam, precision=prec, steps_per_pass=128)
+        (d,), s4 = ctx.render(cam, precision=prec, steps_per_pass=16)
+    waste = [1 - s.composited / s.samples for s in (s1, s2, s3)]
+    print(f"precision {prec}: passes {s1.passes}/{s2.passes}/{s3.passes}, evaluated "
+          f"{s1.samples}/{s2.samples}/{s3.samples}, composited {s1.composited}, "
+          f"waste {waste[0]:.3f}/{waste[1]:.3f}/{waste[2]:.3f}")
+    assert s1.passes == 1 and s2.passes == 2 and s3.passes == 1
+    assert waste[0] > 0.5 and waste[1] < 0.1
+    assert s1.composited == s2.composited == s3.composited
+    assert s2.terminated_rays > 0.2 * 256 * 192
+    for k in ("rgb", "depth", "n_samples", "rgb24"):
+        assert np.array_equal(a[k], b[k]) and np.array_equal(a[k], c[k]) and np.array_equal(a[k], d[k]), k
+
+
+def test_seeded_scene_stays_single_pass(nes, oracle, gpu_ctx):
+    """The seeded scene never reaches T < 1e-4, so the adaptive mode keeps one
+    pass and every evaluated sample is composited."""
+    cam, _ = cams(nes, oracle, 320, 240)
+    for _ in range(2):
+        _, st = gpu_ctx.render(cam, precision=MIXED)
+        assert st.passes == 1 and st.composited == st.samples
+
+
+@pytest.mark.parametrize("prec", [FP32, MIXED])
+def test_budget_renders_in_bounded_passes(nes, oracle, prec):
+    """Translucent full-occupancy scene (about 100 samples per ray): a
+    context whose budget cannot hold the grown sample buffers renders in
+    passes of capacity / rays samples per ray instead, bit-identical to an
+    unbounded context."""
+    scene = _const_scene(1.0)
+    cam, _ = cams(nes, oracle, 320, 240)
+    with nes.Context(*scene) as big:
+        (ref,), sref = big.render(cam, precision=prec)
+    with nes.Context(*scene, hbm_budget=260 << 20) as small:
+        (got,), st = small.render(cam, precision=prec)
+    print(f"precision {prec}: {sref.samples} samples; bounded render {st.passes} passes")
+    assert sref.passes == 1 and st.passes > 2 and st.samples == sref.samples
+    for k in ("rgb", "depth", "n_samples", "rgb24"):
+        assert np.array_equal(ref[k], got[k]), k

@@ -27,7 +27,7 @@ def lpf_recv(sock):
     return exact(n)
 
 
-def test_neserver_protocol_round_trip(nes, gpu_ctx):
+def test_neserver_protocol_round_trip(nes, oracle, gpu_ctx):
     c2w = nes.orbit_camera(30.0, 20.0)
     with nes.WireServer(gpu_ctx, depth_far=4.0) as srv:
         with socket.create_connection(("127.0.0.1", srv.port), timeout=30) as s:
@@ -39,6 +39,6 @@ def test_neserver_protocol_round_trip(nes, gpu_ctx):
                 (ref,), _ = gpu_ctx.render(nes.camera(w, h, c2w, fov=60.0))
                 assert f.frame == ref["rgb24"].tobytes()
                 gray = np.frombuffer(f.depth, np.uint8).reshape(h, w)
-                want = np.clip(np.floor(ref["depth"] / 4.0 * 255.0 + 0.5), 0, 255).astype(np.uint8)
-                assert np.array_equal(gray, want)
+                assert np.array_equal(gray, oracle.depth_gray8(ref["depth"], 4.0))
+                assert np.array_equal(gray, ref["depth_gray8"])
         assert srv.served() == 2
