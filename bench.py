@@ -44,39 +44,136 @@ def time_render(N, ctx, views, precision, steps, warmup):
     return wall, stats
 
 
-def cpu_baseline(N, table, weights, occ, ctx, views, budget_s):
-    """FP64 oracle on the same rays (whole frames, or an even stride of them
-    when that would exceed about budget_s seconds); its pixels double as the
-    PSNR reference of the mixed-precision frames."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O
-    threads = min(16, os.cpu_count() or 1)
-    outs, _ = ctx.render(views, precision=N.PRECISION_MIXED)
-    w, h = views[0].width, views[0].height
-    ocams = [O.camera(w, h, list(v.c2w), fov=v.fov_y_deg) for v in views]
-    probe = np.arange(0, w * h, 97, dtype=np.uint32)
-    t0 = time.perf_counter()
-    O.render_pixels(table, weights, occ, ocams[0], probe, threads=threads)
-    per_ray = (time.perf_counter() - t0) / len(probe)
-    stride = max(1, int(math.ceil(per_ray * w * h * len(views) / budget_s)))
-    total_rays, total_t = 0, 0.0
-    ref, got = [], []
-    for oc, o in zip(ocams, outs):
-        pix = np.arange(0, w * h, stride, dtype=np.uint32)
+def host_cpu():
+    """nproc and the CPU model name of the host the CPU baselines run on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return os.cpu_count() or 1, model
+
+
+def oracle_compare(O, table, weights, occ, views, outs, stride, threads):
+    """FP64 oracle on every stride-th pixel of each view: (rays, seconds,
+    PSNR of outs' float RGB, max |depth error|)."""
+    total_rays, total_t, ref, got, derr = 0, 0.0, [], [], 0.0
+    for v, o in zip(views, outs):
+        oc = O.camera(v.width, v.height, list(v.c2w), fov=v.fov_y_deg)
+        pix = np.arange(0, v.width * v.height, stride, dtype=np.uint32)
         t0 = time.perf_counter()
-        rgb, depth, ns = O.render_pixels(table, weights, occ, oc, pix, threads=threads)
+        rgb, depth, _ = O.render_pixels(table, weights, occ, oc, pix, threads=threads)
         total_t += time.perf_counter() - t0
         total_rays += len(pix)
         ref.append(rgb)
         got.append(o["rgb"].reshape(-1, 3)[pix])
+        derr = max(derr, float(np.abs(o["depth"].reshape(-1)[pix] - depth).max()))
     ref, got = np.concatenate(ref), np.concatenate(got)
     mse = float(np.mean((ref - got) ** 2))
     psnr = 10 * math.log10(1.0 / mse) if mse > 0 else float("inf")
+    return total_rays, total_t, psnr, derr
+
+
+def cpu_baseline(N, table, weights, occ, ctx, views, budget_s):
+    """FP64 oracle on the same rays (whole frames, or an even stride of them
+    when that would exceed about budget_s seconds); its pixels are also the
+    PSNR and depth-error reference of the mixed and FP32 frames."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    nproc, model = host_cpu()
+    threads = min(16, nproc)
+    outs, _ = ctx.render(views, precision=N.PRECISION_MIXED)
+    outs32, _ = ctx.render(views, precision=N.PRECISION_FP32)
+    w, h = views[0].width, views[0].height
+    probe = np.arange(0, w * h, 97, dtype=np.uint32)
+    oc = O.camera(w, h, list(views[0].c2w), fov=views[0].fov_y_deg)
+    t0 = time.perf_counter()
+    O.render_pixels(table, weights, occ, oc, probe, threads=threads)
+    per_ray = (time.perf_counter() - t0) / len(probe)
+    stride = max(1, int(math.ceil(per_ray * w * h * len(views) / budget_s)))
+    rays, secs, psnr, derr = oracle_compare(O, table, weights, occ, views, outs, stride, threads)
+    _, _, psnr32, derr32 = oracle_compare(O, table, weights, occ, views, outs32, stride * 16,
+                                          threads)
     what = "both full 1920x1080 eyes" if stride == 1 else f"every {stride}th pixel of both eyes"
-    base = {"value": total_rays / total_t / 1e6, "unit": "Mrays/s", "cores": threads,
-            "kind": "port",
-            "sample": f"FP64 oracle render of {total_rays} rays ({what}), {total_t:.1f} s"}
-    return base, psnr, total_rays
+    base = {"value": rays / secs / 1e6, "unit": "Mrays/s", "cores": threads, "nproc": nproc,
+            "cpu_model": model, "kind": "port",
+            "sample": f"FP64 oracle render of {rays} rays ({what}), {secs:.1f} s"}
+    acc = {"mixed_psnr_db": round(psnr, 2), "mixed_max_depth_err": derr,
+           "fp32_psnr_db": round(psnr32, 2), "fp32_max_depth_err": derr32,
+           "rays_compared": rays}
+    return base, acc
+
+
+def config1_leg(N, table, weights, occ, ctx, steps=20):
+    """640x480, 60 deg, radius-2 orbit: the plain FP32 path (VALU MLPs), the
+    FP32 MFMA path and the mixed path against the FP64 oracle over the whole
+    frame, plus the oracle's own rate on the host cores."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    nproc, model = host_cpu()
+    threads = min(16, nproc)
+    view = [N.camera(640, 480, N.orbit_camera(30.0, 20.0), fov=60.0)]
+    out = {}
+    for name, prec in (("fp32_valu", N.PRECISION_FP32_VALU), ("fp32", N.PRECISION_FP32),
+                       ("mixed", N.PRECISION_MIXED)):
+        wall, st = time_render(N, ctx, view, prec, steps, 3)
+        outs, _ = ctx.render(view, precision=prec)
+        rays, secs, psnr, derr = oracle_compare(O, table, weights, occ, view, outs, 1, threads)
+        out[name] = {"mrays_per_s": round(640 * 480 * steps / wall / 1e6, 2),
+                     "ms_per_frame": round(wall / steps * 1e3, 4),
+                     "psnr_db": round(psnr, 2), "max_depth_err": derr}
+        out["oracle"] = {"mrays_per_s": round(rays / secs / 1e6, 3), "threads": threads,
+                         "nproc": nproc, "cpu_model": model, "rays": rays}
+    return out
+
+
+def opaque_leg(N, views, steps=10):
+    """Early termination in the timed path: a constant-density opaque scene
+    (sigma = 1500, every cell occupied; each ray stops at T < 1e-4 after its
+    first sample) at stereo 1080p, single pass against the adaptive default
+    (two passes once a frame has shown the waste)."""
+    table = np.full(N.TABLE_ENTRIES * 2, 1.0, np.float32)
+    w = np.zeros(N.MLP_WEIGHTS, np.float32)
+    w[0:2048] = 1.0 / 32.0
+    w[2048:2048 + 64] = math.log(1500.0) / 64.0
+    occ = np.full(N.OCC_WORDS, 0xFFFFFFFF, np.uint32)
+    out = {}
+    with N.Context(table, w, occ) as c:
+        for name, spp in (("single_pass", 128), ("adaptive", 0)):
+            for _ in range(2):
+                c.render_device(views, N.PRECISION_MIXED, spp)
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                st = c.render_device(views, N.PRECISION_MIXED, spp)
+            ms = (time.perf_counter() - t0) / steps * 1e3
+            out[name] = {"ms_per_frame": round(ms, 4), "passes": st.passes,
+                         "evaluated": st.samples, "composited": st.composited,
+                         "waste": round(1 - st.composited / max(st.samples, 1), 4)}
+    return out
+
+
+def ceilings_leg(N, samples, ms_encode, l2_req_per_launch):
+    """Measured gather ceilings (nes_device_gather_rate): random one-lane
+    4-byte loads inside 16 KiB per XCD (vector-L1 hits: lane-address rate) and
+    inside 2 MiB per XCD (L1 misses, L2 hits: L2 request rate), and the
+    encode's rates against them."""
+    l1, _ = N.gather_rate(16 << 10)
+    l2, _ = N.gather_rate(2 << 20)
+    lane_loads = 128 * samples / (ms_encode * 1e-3)  # 16 levels x 8 corners
+    out = {"l1_lane_loads_per_s": l1, "l2_requests_per_s": l2,
+           "encode_lane_loads_per_s": lane_loads, "encode_l1_frac": round(lane_loads / l1, 4)}
+    if l2_req_per_launch:
+        rate = l2_req_per_launch / (ms_encode * 1e-3)
+        out.update({"encode_l2_requests_per_s": rate, "encode_l2_frac": round(rate / l2, 4),
+                    # level 15 on XCD 0: no reuse between samples, so 4 L2
+                    # requests (one per y/z corner pair) per sample
+                    "xcd0_l2_requests_per_s": 4 * samples / (ms_encode * 1e-3),
+                    "xcd0_l2_frac": round(4 * samples / (ms_encode * 1e-3) / (l2 / 8), 4)})
+    return out
 
 
 def cpu_frameprep_baseline(threads, w=1280, h=720, runs=100, warmup=10):
@@ -173,14 +270,12 @@ def training_leg(N, ctx, occ, steps=12, warmup=3):
             "loss_first": losses[0], "loss_last": losses[-1]}
 
 
-def load_traffic():
-    """HBM bytes per launch of encode_parts<f16> on this workload, from the
-    committed rocprofv3 PMC summary (FETCH_SIZE + WRITE_SIZE, calibrated
-    against a plain device copy of known size)."""
-    p = os.path.join(ROOT, "profiles", "encode_parts_f16_stereo1080p_pmc.json")
+def load_profile(name, key):
+    """A figure from a committed rocprofv3 summary under profiles/."""
+    p = os.path.join(ROOT, "profiles", name)
     if os.path.exists(p):
         with open(p) as f:
-            return json.load(f).get("hbm_bytes_per_launch")
+            return json.load(f).get(key)
     return None
 
 
@@ -195,8 +290,12 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--skip-serve", action="store_true")
     ap.add_argument("--skip-train", action="store_true")
+    ap.add_argument("--skip-extra", action="store_true",
+                    help="skip the config-1, opaque-scene and ceiling legs")
     ap.add_argument("--train-only", action="store_true",
                     help="only the training leg (for profiling its kernels)")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="only the serving leg (for profiling its kernels)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 device copies of 1 GiB (memory-counter calibration)")
     args = ap.parse_args()
@@ -223,22 +322,41 @@ def main():
             print(json.dumps({"training": training_leg(N, ctx, occ, steps=args.steps,
                                                        warmup=args.warmup)}))
             return
+        if args.serve_only:
+            print(json.dumps({"serving": serving_leg(N, ctx, args.serve_seconds)}))
+            return
         wall_m, st_m = time_render(N, ctx, views, N.PRECISION_MIXED, args.steps, args.warmup)
         log(f"mixed: {wall_m / args.steps * 1e3:.3f} ms/step")
-        wall_f, st_f = time_render(N, ctx, views, N.PRECISION_FP32, max(3, args.steps // 4),
-                                   args.warmup)
-        steps_f = len(st_f)
-        log(f"fp32: {wall_f / steps_f * 1e3:.3f} ms/step")
+        nf = max(3, args.steps // 4)
+        wall_f, st_f = time_render(N, ctx, views, N.PRECISION_FP32, nf, args.warmup)
+        log(f"fp32 (MFMA MLPs): {wall_f / nf * 1e3:.3f} ms/step")
+        wall_v, st_v = time_render(N, ctx, views, N.PRECISION_FP32_VALU, nf, args.warmup)
+        log(f"fp32 (VALU MLPs): {wall_v / nf * 1e3:.3f} ms/step")
         samples = st_m[-1].samples
-        ms_eval = float(np.mean([s.ms_eval for s in st_m]))
         ms_enc = float(np.mean([s.ms_encode for s in st_m]))
         ms_mlp = float(np.mean([s.ms_mlp for s in st_m]))
         ms_march = float(np.mean([s.ms_march for s in st_m]))
         ms_comp = float(np.mean([s.ms_composite for s in st_m]))
         value = rays * args.steps / wall_m / 1e6
-        fp32 = rays * steps_f / wall_f / 1e6
+        fp32 = rays * nf / wall_f / 1e6
+        fp32_valu = rays * nf / wall_v / 1e6
         gathered_gbs = GATHER_BYTES_PER_SAMPLE * samples / (ms_enc * 1e-3) / 1e9
         mlp_tflops = FLOP_PER_SAMPLE * samples / (ms_mlp * 1e-3) / 1e12
+        traffic = load_profile("encode_parts_f16_stereo1080p_pmc.json", "hbm_bytes_per_launch")
+        roofline = {
+            "kernel": "encode_parts<f16>", "bound": "hbm",
+            "achieved": round(gathered_gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": round(gathered_gbs / PEAK_HBM_GBS, 4), "traffic": traffic,
+            # what the counters show: the gathers are L1/L2 hits; XCD 0 (level
+            # 15, no reuse) runs at the L2 request rate, the others at the
+            # vector-L1 lane-address rate; HBM carries far less than 512 B/sample
+            "limiter": "L2 request rate of XCD 0 (level 15) and vector-L1 lane-address rate "
+                       "(XCDs 1-7); HBM is not the limiter",
+        }
+        if traffic:
+            hbm = traffic / (ms_enc * 1e-3) / 1e9
+            roofline.update({"hbm_measured_GBs": round(hbm, 1),
+                             "hbm_measured_frac": round(hbm / PEAK_HBM_GBS, 4)})
         result = {
             "metric": METRIC, "value": round(value, 3), "unit": "Mrays/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup,
@@ -249,37 +367,47 @@ def main():
                                    "precision: FP16 table, FP16 MFMA MLPs, FP32 compositing",
                        "rays_per_step": rays, "samples_per_step": int(samples),
                        "samples_per_ray": round(samples / rays, 4)},
-            "fp32_mrays": round(fp32, 3),
+            "fp32_mrays": round(fp32, 3), "fp32_valu_mrays": round(fp32_valu, 3),
             "speedup_vs_fp32": round(value / fp32, 3),
+            "speedup_vs_fp32_valu": round(value / fp32_valu, 3),
             "stage_ms": {"march": round(ms_march, 4), "encode": round(ms_enc, 4),
                          "mlp": round(ms_mlp, 4), "composite": round(ms_comp, 4),
                          "fp32_encode": round(float(np.mean([s.ms_encode for s in st_f])), 4),
-                         "fp32_mlp": round(float(np.mean([s.ms_mlp for s in st_f])), 4)},
-            # dominant kernel: the FP16 hash-grid encode (512 B gathered per sample)
-            "roofline": {"kernel": "encode_parts<f16>", "bound": "hbm",
-                         "achieved": round(gathered_gbs, 1), "peak": PEAK_HBM_GBS,
-                         "unit": "GB/s", "frac": round(gathered_gbs / PEAK_HBM_GBS, 4),
-                         "traffic": load_traffic()},
+                         "fp32_mlp": round(float(np.mean([s.ms_mlp for s in st_f])), 4),
+                         "fp32_valu_mlp": round(float(np.mean([s.ms_mlp for s in st_v])), 4)},
+            "roofline": roofline,
             "mlp_roofline": {"kernel": "mlp_mixed", "bound": "mfma",
                              "achieved": round(mlp_tflops, 2), "peak": PEAK_FP16_TFLOPS,
                              "unit": "TFLOP/s", "frac": round(mlp_tflops / PEAK_FP16_TFLOPS, 5)},
-            "psnr_db": None, "p99_frame_latency_ms": None, "cpu_baseline": None,
+            "psnr_db": None, "max_depth_err": None, "p99_frame_latency_ms": None,
+            "cpu_baseline": None,
         }
+        if not args.skip_extra:
+            result["ceilings"] = ceilings_leg(
+                N, samples, ms_enc, load_profile("encode_parts_f16_stereo1080p_l2.json",
+                                                 "l2_requests_per_launch"))
+            result["config1_640x480"] = config1_leg(N, table, weights, occ, ctx)
+            result["opaque_stereo1080p"] = opaque_leg(N, views)
         if not args.skip_frameprep:
             result["frameprep"] = frameprep_leg(N)
         if not args.skip_train:
             result["training"] = training_leg(N, ctx, occ)
         if not args.skip_serve:
             srv = serving_leg(N, ctx, args.serve_seconds)
+            prof = load_profile("serving_8clients_roofline.json", "summary")
+            if prof:
+                srv["rocprof"] = prof
             result["serving"] = srv
             result["p99_frame_latency_ms"] = round(srv["p99_ms"], 3)
         if not args.skip_cpu:
-            base, psnr, nr = cpu_baseline(N, table, weights, occ, ctx, views, args.cpu_seconds)
+            base, acc = cpu_baseline(N, table, weights, occ, ctx, views, args.cpu_seconds)
             result["cpu_baseline"] = base
-            result["psnr_db"] = round(psnr, 2)
-            result["psnr_rays"] = nr
+            result["psnr_db"] = acc["mixed_psnr_db"]
+            result["max_depth_err"] = acc["mixed_max_depth_err"]
+            result["accuracy_vs_fp64"] = acc
             if "frameprep" in result:
                 cf = cpu_frameprep_baseline(base["cores"])
+                cf["nproc"], cf["cpu_model"] = base["nproc"], base["cpu_model"]
                 result["frameprep"]["cpu"] = cf
                 result["frameprep"]["gpu_vs_cpu_1t"] = round(
                     cf["ms_per_frame_1t"] / (result["frameprep"]["ms_per_launch"] /

@@ -475,6 +475,12 @@ int nes_memcpy_d2h(void *dst, const void *src, size_t bytes);
 /* Device-to-device copy (a plain copy of known size, used to calibrate
  * memory counters); elapsed_ms optional. */
 int nes_device_copy(void *dst, const void *src, size_t bytes, float *elapsed_ms);
+/* Memory-hierarchy ceiling probe: random independent 4-byte gathers, one
+ * address per lane, each XCD within its own region_bytes (a power of two,
+ * 4 KiB .. 64 MiB).  A region inside the vector L1 gives the chip's L1
+ * lane-address rate, one inside an XCD's L2 (but not L1) its L2 request
+ * rate: lane_loads / elapsed_ms of the timed launch. */
+int nes_device_gather_rate(size_t region_bytes, float *elapsed_ms, uint64_t *lane_loads);
 int nes_device_info(char *name, size_t name_len, int *cu_count,
                     uint64_t *hbm_bytes);
 

@@ -72,6 +72,27 @@ void DevBuf::release() {
   bytes = 0;
 }
 
+// Gather-rate probe: random, independent 4-byte loads (one address per lane)
+// within a per-XCD region of the buffer -- workgroup b runs on XCD b % 8 and
+// reads region b % 8.  A region that fits the vector L1 measures the lane
+// address rate of L1 hits; one that fits an XCD's L2 but not L1 measures the
+// L2 request rate (every lane load misses L1).
+__global__ __launch_bounds__(256) void gather_probe(const uint32_t *__restrict__ buf,
+                                                    uint32_t region_mask, uint32_t region_words,
+                                                    uint32_t iters, uint32_t *sink) {
+  const uint32_t *base = buf + (size_t)(blockIdx.x & 7u) * region_words;
+  uint32_t h = (blockIdx.x * blockDim.x + threadIdx.x + 1u) * 0x9E3779B1u;
+  uint32_t acc = 0;
+  for (uint32_t i = 0; i < iters; i += 8) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      h = h * 1664525u + 1013904223u;
+      acc ^= __ldg(base + ((h >> 8) & region_mask));
+    }
+  }
+  if (acc == 0x9E3779B9u) sink[threadIdx.x] = acc;  // keeps the loads live
+}
+
 }  // namespace nes
 
 extern "C" {
@@ -150,6 +171,39 @@ int nes_device_copy(void *dst, const void *src, size_t bytes, float *elapsed_ms)
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   return NES_OK;
+}
+
+int nes_device_gather_rate(size_t region_bytes, float *elapsed_ms, uint64_t *lane_loads) {
+  if (region_bytes < 4096 || region_bytes > (64u << 20) || (region_bytes & (region_bytes - 1)))
+    return NES_EINVAL;
+  NES_TRY(nes::ensure_device());
+  hipStream_t s = nes::library_stream();
+  const uint32_t words = (uint32_t)(region_bytes / 4);
+  constexpr uint32_t kBlocks = 8 * 2048, kIters = 256;
+  uint32_t *buf = nullptr, *sink = nullptr;
+  NES_HIP_TRY(hipMalloc(&buf, 8 * region_bytes));
+  if (hipMalloc(&sink, 1024) != hipSuccess) {
+    hipFree(buf);
+    return NES_ENOMEM;
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = NES_OK;
+  if (hipMemsetAsync(buf, 0, 8 * region_bytes, s) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+      hipEventCreate(&e1) != hipSuccess)
+    rc = NES_EIO;
+  for (int run = 0; rc == NES_OK && run < 2; ++run) {  // the first launch warms the caches
+    hipEventRecord(e0, s);
+    nes::gather_probe<<<kBlocks, 256, 0, s>>>(buf, words - 1, words, kIters, sink);
+    hipEventRecord(e1, s);
+    if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess) rc = NES_EIO;
+  }
+  if (rc == NES_OK && elapsed_ms) hipEventElapsedTime(elapsed_ms, e0, e1);
+  if (lane_loads) *lane_loads = (uint64_t)kBlocks * 256 * kIters;
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  hipFree(sink);
+  hipFree(buf);
+  return rc;
 }
 
 int nes_device_info(char *name, size_t name_len, int *cu_count,

@@ -210,6 +210,7 @@ _sig("nes_memcpy_h2d", C.c_int, C.c_void_p, C.c_void_p, C.c_size_t)
 _sig("nes_memcpy_d2h", C.c_int, C.c_void_p, C.c_void_p, C.c_size_t)
 _sig("nes_device_copy", C.c_int, C.c_void_p, C.c_void_p, C.c_size_t, _P(C.c_float))
 _sig("nes_device_info", C.c_int, C.c_char_p, C.c_size_t, _P(C.c_int), _P(C.c_uint64))
+_sig("nes_device_gather_rate", C.c_int, C.c_size_t, _P(C.c_float), _P(C.c_uint64))
 
 _lib = None
 
@@ -528,6 +529,16 @@ def device_copy(dst, src, nbytes):
     ms = C.c_float()
     check(lib().nes_device_copy(dst.ptr, src.ptr, nbytes, C.byref(ms)), "nes_device_copy")
     return ms.value
+
+
+def gather_rate(region_bytes):
+    """Random one-address-per-lane 4-byte gathers within a per-XCD region:
+    (lane loads per second, kernel ms)."""
+    ms = C.c_float()
+    n = C.c_uint64()
+    check(lib().nes_device_gather_rate(region_bytes, C.byref(ms), C.byref(n)),
+          "nes_device_gather_rate")
+    return n.value / (ms.value * 1e-3), ms.value
 
 
 def device_info():

@@ -240,15 +240,16 @@ def test_seeded_scene_stays_single_pass(nes, oracle, gpu_ctx):
 
 @pytest.mark.parametrize("prec", [FP32, MIXED])
 def test_budget_renders_in_bounded_passes(nes, oracle, prec):
-    """Translucent full-occupancy scene (about 100 samples per ray): a
-    context whose budget cannot hold the grown sample buffers renders in
+    """Translucent full-occupancy scene (~3.5 M samples at 640 x 480, the
+    initial buffers hold 1.2 M): a context whose budget cannot hold the
+    grown sample buffers renders in
     passes of capacity / rays samples per ray instead, bit-identical to an
     unbounded context."""
     scene = _const_scene(1.0)
-    cam, _ = cams(nes, oracle, 320, 240)
+    cam, _ = cams(nes, oracle, 640, 480)
     with nes.Context(*scene) as big:
         (ref,), sref = big.render(cam, precision=prec)
-    with nes.Context(*scene, hbm_budget=260 << 20) as small:
+    with nes.Context(*scene, hbm_budget=300 << 20) as small:
         (got,), st = small.render(cam, precision=prec)
     print(f"precision {prec}: {sref.samples} samples; bounded render {st.passes} passes")
     assert sref.passes == 1 and st.passes > 2 and st.samples == sref.samples
