@@ -360,12 +360,10 @@ __global__ __launch_bounds__(kFpThreads) void frameprep_kernel(FrameprepArgs a) 
   }
 }
 
-int launch_frameprep(const uint8_t *rgb_a, const float *depth_a,
-                     const uint8_t *rgb_b, const float *depth_b, int w, int h,
-                     int n_frames, size_t rgb_fb, size_t depth_fb, uint8_t *y,
-                     int y_stride, size_t y_fb, uint8_t *u, int u_stride,
-                     size_t u_fb, uint8_t *v, int v_stride, size_t v_fb,
-                     int sws_mode, hipStream_t stream) {
+static int frameprep_check(const uint8_t *rgb_a, const float *depth_a, const uint8_t *rgb_b,
+                           const float *depth_b, int w, int h, int n_frames, const uint8_t *y,
+                           int y_stride, const uint8_t *u, int u_stride, const uint8_t *v,
+                           int v_stride, int sws_mode) {
   if (w < 2 || h < 2 || (w & 1) || (h & 1) || n_frames < 1 || !rgb_a || !y || !u || !v) {
     set_last_error("frame prep needs even w, h >= 2 and non-null buffers");
     return NES_EINVAL;
@@ -377,8 +375,38 @@ int launch_frameprep(const uint8_t *rgb_a, const float *depth_a,
   if (y_stride < w || u_stride < w / 2 || v_stride < w / 2) return NES_EINVAL;
   if (sws_mode != NES_SWS_X86 && sws_mode != NES_SWS_C) return NES_EINVAL;
   if (n_frames > 65535) return NES_ERANGE;
+  return NES_OK;
+}
+
+// The frame-prep launch with the vertical chroma filter `flt` (device table).
+static int launch_frameprep_filter(const uint8_t *rgb_a, const float *depth_a,
+                                   const uint8_t *rgb_b, const float *depth_b, int w, int h,
+                                   int n_frames, size_t rgb_fb, size_t depth_fb, uint8_t *y,
+                                   int y_stride, size_t y_fb, uint8_t *u, int u_stride,
+                                   size_t u_fb, uint8_t *v, int v_stride, size_t v_fb,
+                                   int sws_mode, const DevVFilter &flt, hipStream_t stream);
+
+int launch_frameprep(const uint8_t *rgb_a, const float *depth_a,
+                     const uint8_t *rgb_b, const float *depth_b, int w, int h,
+                     int n_frames, size_t rgb_fb, size_t depth_fb, uint8_t *y,
+                     int y_stride, size_t y_fb, uint8_t *u, int u_stride,
+                     size_t u_fb, uint8_t *v, int v_stride, size_t v_fb,
+                     int sws_mode, hipStream_t stream) {
+  NES_TRY(frameprep_check(rgb_a, depth_a, rgb_b, depth_b, w, h, n_frames, y, y_stride, u,
+                          u_stride, v, v_stride, sws_mode));
   DevVFilter flt;
   NES_TRY(get_dev_vfilter(h, &flt));
+  return launch_frameprep_filter(rgb_a, depth_a, rgb_b, depth_b, w, h, n_frames, rgb_fb,
+                                 depth_fb, y, y_stride, y_fb, u, u_stride, u_fb, v, v_stride,
+                                 v_fb, sws_mode, flt, stream);
+}
+
+static int launch_frameprep_filter(const uint8_t *rgb_a, const float *depth_a,
+                                   const uint8_t *rgb_b, const float *depth_b, int w, int h,
+                                   int n_frames, size_t rgb_fb, size_t depth_fb, uint8_t *y,
+                                   int y_stride, size_t y_fb, uint8_t *u, int u_stride,
+                                   size_t u_fb, uint8_t *v, int v_stride, size_t v_fb,
+                                   int sws_mode, const DevVFilter &flt, hipStream_t stream) {
   FrameprepArgs a;
   a.rgb_a = rgb_a; a.depth_a = depth_a; a.rgb_b = rgb_b; a.depth_b = depth_b;
   a.y = y; a.u = u; a.v = v; a.coef = flt.coef; a.pos = flt.pos;
