@@ -40,14 +40,21 @@ constexpr uint32_t kMaxSlots = 1u << 25;
 constexpr uint32_t kCountMask = 0xffffu;
 
 // Words of RenderWork::total (device) and RenderWork::total_host (pinned).
-//   device: [0] sample cursor of the current pass, [1] samples composited,
-//           [2] rays with at least one composited sample, [3] rays that
-//           reached T < 1e-4, [4] samples composited by those rays,
-//           [5] samples evaluated by all passes of the frame
+//   device: [0] sample cursor of the current pass, [1] samples evaluated by
+//           all passes of the frame (added once per pass), then from word
+//           kTotTermBase kTotTermSlots copies of the termination counters,
+//           one 128-byte line each (a wavefront adds to copy wave % slots,
+//           and only when one of its rays reached T < 1e-4 in the pass):
+//           [+0] rays that reached T < 1e-4, [+1] samples those rays
+//           composited, [+2] samples evaluated behind that point,
+//           [+3] rays that stopped before the last pass with steps left
 //   host:   [0] first-pass sample total, [1] later-pass sample total (0 for
-//           a single pass), [2..6] device words [1..5] of the frame
-constexpr int kTotCursor = 0, kTotComposited = 1, kTotHitRays = 2, kTotTerminated = 3,
-              kTotTermSamples = 4, kTotEvaluated = 5, kTotWords = 6;
+//           a single pass), [2] device word [1], and from kTotTermBase the
+//           device's counter copies
+constexpr int kTotCursor = 0, kTotEvaluated = 1, kTotTermBase = 32, kTotTermSlots = 32,
+              kTotTermStride = 32;
+constexpr int kTotWords = kTotTermBase + kTotTermSlots * kTotTermStride;
+constexpr size_t kTotBytes = sizeof(uint32_t) * kTotWords;
 
 struct RenderWork {
   DevBuf views, rays, acc, counts, offsets, samples, sample_out, sample_pos, feat, out, total;
@@ -70,7 +77,8 @@ struct RenderWork {
 
 // What one frame evaluated and composited (for early-termination stats).
 struct FrameCounts {
-  uint64_t evaluated = 0, composited = 0, hit_rays = 0, terminated = 0;
+  uint64_t evaluated = 0, composited = 0, terminated = 0, term_samples = 0, wasted = 0,
+           saved_rays = 0;
   uint32_t passes = 1;
 };
 

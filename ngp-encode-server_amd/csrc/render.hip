@@ -151,10 +151,7 @@ __global__ __launch_bounds__(256) void march_emit(MarchArgs m) {
   }
   const uint32_t wave_total = wave_sum(cnt);
   uint32_t base = 0;
-  if (lane == 0 && wave_total) {
-    base = atomicAdd(m.cursor + kTotCursor, wave_total);
-    atomicAdd(m.cursor + kTotEvaluated, wave_total);
-  }
+  if (lane == 0 && wave_total) base = atomicAdd(m.cursor + kTotCursor, wave_total);
   base = __shfl(base, 0);
   if (slot < n_slots) {
     counts[slot] = cnt | (knext << 16);
@@ -469,6 +466,7 @@ __global__ __launch_bounds__(256) void composite(
   base = __shfl(base, 0);  // lane 0 is valid whenever any lane is
   const bool live = valid && !a.done && cnt;
   if (live && !tinfo) ts = load_ray(rays, slot).t_start;
+  const uint32_t n_before = a.n_samples;
   const uint32_t mx = wave_max(cnt);
   const uint64_t below = lanes_below(lane);
   uint32_t before = 0;
@@ -508,23 +506,30 @@ __global__ __launch_bounds__(256) void composite(
       }
     }
   }
-  if (last && stats) {
-    // frame counters: samples composited, rays that composited any, rays
-    // that reached T < 1e-4 and the samples those composited
-    const bool px = valid && a.pad != 0xffffffffu;
-    const bool term = px && a.n_samples > 0 && a.T < kTStop;
-    const uint32_t nc = wave_sum(px ? a.n_samples : 0u);
-    const uint32_t nt = wave_sum(term ? a.n_samples : 0u);
-    const uint32_t nh = (uint32_t)__popcll(__ballot(px && a.n_samples > 0));
-    const uint32_t nr = (uint32_t)__popcll(__ballot(term));
-    if (lane == 0 && nc) {
-      atomicAdd(stats + kTotComposited, nc);
-      atomicAdd(stats + kTotHitRays, nh);
-      if (nr) {
-        atomicAdd(stats + kTotTerminated, nr);
-        atomicAdd(stats + kTotTermSamples, nt);
+  if (stats) {
+    // Termination counters, touched only by wavefronts with a ray that
+    // reached T < 1e-4 in this pass (none in a scene that never does), on
+    // one of kTotTermSlots lines so that such wavefronts rarely share one.
+    const bool term = live && a.done;
+    const uint64_t tb = __ballot(term);
+    if (tb) {
+      const uint32_t here = a.n_samples - n_before;  // composited in this pass
+      bool saved = false;
+      if (term && !last) saved = (counts[slot] >> 16) < load_ray(rays, slot).n_steps;
+      const uint32_t ts_sum = wave_sum(term ? a.n_samples : 0u);
+      const uint32_t waste = wave_sum(term ? cnt - here : 0u);
+      const uint32_t ns = (uint32_t)__popcll(__ballot(saved));
+      if (lane == 0) {
+        uint32_t *c = stats + kTotTermBase +
+                      kTotTermStride * (((blockIdx.x * blockDim.x + threadIdx.x) >> 6) % kTotTermSlots);
+        atomicAdd(c + 0, (uint32_t)__popcll(tb));
+        atomicAdd(c + 1, ts_sum);
+        if (waste) atomicAdd(c + 2, waste);
+        if (ns) atomicAdd(c + 3, ns);
       }
     }
+    // every pass adds its sample total once
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(stats + kTotEvaluated, *n_ptr);
   }
   if (!valid) return;
   // the state carries into the next pass; after the last pass only a caller
@@ -703,8 +708,8 @@ int render_setup(RenderWork &w, const nes_camera *views, int n_views, hipStream_
   NES_TRY(w.ray_sh.reserve(32 * slots));
   NES_TRY(w.tinfo.reserve(8 * slots));
   NES_TRY(w.out.reserve(out_layout(px).total));
-  NES_TRY(w.total.reserve(64));
-  if (!w.total_host) NES_HIP_TRY(hipHostMalloc((void **)&w.total_host, 64));
+  NES_TRY(w.total.reserve(kTotBytes));
+  if (!w.total_host) NES_HIP_TRY(hipHostMalloc((void **)&w.total_host, kTotBytes));
   for (hipEvent_t &e : w.ev)
     if (!e) NES_HIP_TRY(hipEventCreate(&e));
   NES_HIP_TRY(hipMemcpyAsync(w.views.ptr, w.host_views.data(), sizeof(ViewDesc) * n_views,
@@ -760,7 +765,7 @@ int render_launch_pass(const SceneDev &scene, RenderWork &w, int precision, int 
   // a render in one pass keeps no per-ray state between march and composite
   uint2 *tinfo = first && last ? w.tinfo.as<uint2>() : nullptr;
   // the frame counters start with the first pass, the cursor with every pass
-  zero_u32<<<1, 64, 0, stream>>>(w.total.as<uint32_t>(), first ? kTotWords : 1);
+  zero_u32<<<(kTotWords + 255) / 256, 256, 0, stream>>>(w.total.as<uint32_t>(), first ? kTotWords : 1);
   MarchArgs m = march_args(w, scene.occ, k0, k1, first, ray_cap, resume);
   m.packed = precision == NES_PRECISION_MIXED ? 1 : 0;
   m.ray_sh = precision == NES_PRECISION_MIXED ? w.ray_sh.as<uint2>() : nullptr;
@@ -811,9 +816,12 @@ int render_launch_pass(const SceneDev &scene, RenderWork &w, int precision, int 
   NES_HIP_TRY(hipGetLastError());
   NES_HIP_TRY(hipMemcpyAsync(w.total_host + (first ? 0 : 1), cursor, 4, hipMemcpyDeviceToHost,
                              stream));
-  if (last)
-    NES_HIP_TRY(hipMemcpyAsync(w.total_host + 2, stats + kTotComposited, 4 * (kTotWords - 1),
-                               hipMemcpyDeviceToHost, stream));
+  if (last) {
+    NES_HIP_TRY(hipMemcpyAsync(w.total_host + 2, stats + kTotEvaluated, 4, hipMemcpyDeviceToHost,
+                               stream));
+    NES_HIP_TRY(hipMemcpyAsync(w.total_host + kTotTermBase, stats + kTotTermBase,
+                               4 * (kTotWords - kTotTermBase), hipMemcpyDeviceToHost, stream));
+  }
   return NES_OK;
 }
 
@@ -881,20 +889,27 @@ size_t sample_buffer_bytes(size_t capacity, int precision) {
 // pass evaluates every occupied sample; when more than 5 % of them lie behind
 // the point where T fell below 1e-4, the next frame runs two passes, the first
 // giving each ray as many samples as the terminated rays needed on average
-// (rounded up).  Two-pass frames go back to one pass when fewer than 5 % of
-// the rays with samples terminate early.
+// (rounded up).  Multi-pass frames go back to one pass when the first pass
+// stopped fewer than 1 % of the pixels with steps still ahead of them.
 FrameCounts render_frame_finish(RenderWork &w) {
   FrameCounts c;
   c.passes = w.frame_passes;
-  c.evaluated = w.total_host[6];
-  c.composited = w.total_host[2];
-  c.hit_rays = w.total_host[3];
-  c.terminated = w.total_host[4];
-  const uint64_t term_samples = w.total_host[5];
-  const bool wasteful = c.evaluated > c.composited && (c.evaluated - c.composited) * 20 > c.evaluated;
-  const bool terminating = c.hit_rays > 0 && c.terminated * 20 > c.hit_rays;
-  if ((c.passes == 1 && wasteful) || (c.passes > 1 && terminating)) {
-    const uint64_t m = (term_samples + c.terminated - 1) / std::max<uint64_t>(c.terminated, 1);
+  c.evaluated = w.total_host[2];
+  for (int s = 0; s < kTotTermSlots; ++s) {
+    const uint32_t *t = w.total_host + kTotTermBase + kTotTermStride * s;
+    c.terminated += t[0];
+    c.term_samples += t[1];
+    c.wasted += t[2];
+    c.saved_rays += t[3];
+  }
+  c.composited = c.evaluated - std::min(c.wasted, c.evaluated);
+  // one pass: switch to two when more than 5 % of the evaluated samples lay
+  // behind T < 1e-4; several passes: keep two while more than 1 % of the
+  // pixels stopped early with steps left (the work the first pass saved)
+  const bool wasteful = c.wasted * 20 > c.evaluated;
+  const bool saving = c.saved_rays * 100 > w.out_px;
+  if (c.terminated && ((c.passes == 1 && wasteful) || (c.passes > 1 && saving))) {
+    const uint64_t m = (c.term_samples + c.terminated - 1) / c.terminated;
     w.first_pass_cap = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(m, 1), kMaxSteps - 1);
   } else {
     w.first_pass_cap = kMaxSteps;
@@ -906,7 +921,7 @@ FrameCounts render_frame_finish(RenderWork &w) {
 // state already initialised by the caller, one output index per ray slot).
 int launch_march_emit(const uint32_t *occ, RenderWork &w, int k0, int k1, hipStream_t stream) {
   const unsigned blocks = (unsigned)((w.n_slots + 255) / 256);
-  zero_u32<<<1, 64, 0, stream>>>(w.total.as<uint32_t>(), kTotWords);
+  zero_u32<<<1, 64, 0, stream>>>(w.total.as<uint32_t>(), 2);
   march_emit<<<blocks, 256, 0, stream>>>(march_args(w, occ, k0, k1, false, kMaxSteps, false));
   NES_HIP_TRY(hipGetLastError());
   return NES_OK;
@@ -1073,13 +1088,11 @@ static int render_enqueue_inner(const SceneDev &scene, RenderWork &w, int precis
         }
         NES_TRY(render_reserve_samples(w, precision, want));
       }
-      fc.evaluated += n;
       add_pass_times(ev, end, t);
     }
-    fc.composited = w.total_host[2];
-    fc.hit_rays = w.total_host[3];
-    fc.terminated = w.total_host[4];
-    fc.passes = (uint32_t)passes;
+    w.frame_passes = (uint32_t)passes;
+    fc = render_frame_finish(w);
+    w.first_pass_cap = kMaxSteps;  // windows are the caller's choice, not learned from
   }
   float total = 0;
   hipEventElapsedTime(&total, w.ev[0], end);

@@ -604,9 +604,9 @@ static int trainer_prepare(nes_trainer *t, uint32_t n_rays) {
   NES_TRY(w.counts.reserve(sizeof(uint32_t) * n_rays));
   NES_TRY(w.offsets.reserve(sizeof(uint32_t) * n_rays));
   NES_TRY(w.out.reserve(out_layout(n_rays).total));
-  NES_TRY(w.total.reserve(64));
+  NES_TRY(w.total.reserve(kTotBytes));
   NES_TRY(w.views.reserve(sizeof(ViewDesc)));
-  if (!w.total_host) NES_HIP_TRY(hipHostMalloc((void **)&w.total_host, 64));
+  if (!w.total_host) NES_HIP_TRY(hipHostMalloc((void **)&w.total_host, kTotBytes));
   if (w.capacity < (size_t)n_rays * 4 || w.capacity_precision != NES_PRECISION_FP32)
     NES_TRY(render_reserve_samples(w, NES_PRECISION_FP32, std::max<size_t>(w.capacity, (size_t)n_rays * 4)));
   return NES_OK;
