@@ -179,6 +179,23 @@ int nes_frameprep_device(const uint8_t *rgb_a, const float *depth_a,
                          uint8_t *u, int u_stride, uint8_t *v, int v_stride,
                          int sws_mode, float *elapsed_ms);
 
+/* nes_frameprep_device without device allocation.  This entry point has no
+ * counterpart in the reference: it is for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Its only scratch buffer, the vertical chroma filter of
+ * the frame height, lives in a fixed device array of 144 KiB per GPU, held
+ * until the process ends, so h is limited to 8192 (NES_ERANGE above).  The
+ * array is shared by every call on a GPU and keeps the last call's table:
+ * calls that overlap on one GPU race.  The work is enqueued on `stream` (a
+ * hipStream_t) and the call returns without waiting; stream NULL uses the
+ * null stream and returns after completion.  Same layout and results as
+ * nes_frameprep_device. */
+int nes_frameprep_device_static(const uint8_t *rgb_a, const float *depth_a,
+                                const uint8_t *rgb_b, const float *depth_b, int w,
+                                int h, int n_frames, uint8_t *y, int y_stride,
+                                uint8_t *u, int u_stride, uint8_t *v, int v_stride,
+                                int sws_mode, void *stream);
+
 /* GRAY8 (full range) -> YUV420P, the depth stream of
  * RenderedFrame::convert_frame (include/base/video/rendered_frame.h:30-31). */
 int nes_gray8_to_yuv420p(const uint8_t *gray, int w, int h, int g_stride,

@@ -501,6 +501,62 @@ std::mutex g_host_mu;
 DevBuf g_in, g_out;
 }  // namespace
 
+// ------------------------------------------------ allocation-free variant
+// nes_frameprep_device_static keeps its one scratch buffer -- the vertical
+// chroma filter of the frame height (coefficients, then row positions) -- in
+// this fixed device array instead of the per-height device cache, so a call
+// allocates no device memory.  Each GPU holds its own copy until the process
+// ends; the host refers to it only through the address hipGetSymbolAddress
+// returns for the current device.
+constexpr int kStaticMaxHeight = 8192;
+constexpr int kStaticMaxTaps = 16;
+constexpr size_t kStaticCoefBytes = (size_t)(kStaticMaxHeight / 2) * kStaticMaxTaps * sizeof(int16_t);
+constexpr size_t kStaticScratchBytes = kStaticCoefBytes + (size_t)(kStaticMaxHeight / 2) * sizeof(int32_t);
+static __device__ __align__(256) uint8_t scratch[kStaticScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_static_mu;
+void *g_scratch_dev[kMaxDevices] = {};        // device address of `scratch` per device
+std::map<int, VFilter> g_static_filters;       // host filter tables by height
+}  // namespace
+
+// The current device's address of `scratch`, looked up on the first call on
+// each device with that device current.
+static int static_scratch(void **out) {
+  int dev = 0;
+  NES_HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return NES_ENODEV;
+  std::lock_guard<std::mutex> lk(g_static_mu);
+  if (!g_scratch_dev[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_last_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+      return NES_EIO;
+    }
+    g_scratch_dev[dev] = p;
+  }
+  *out = g_scratch_dev[dev];
+  return NES_OK;
+}
+
+// Host filter table of height h (built once per height; host memory only).
+static int static_filter(int h, const VFilter **out) {
+  std::lock_guard<std::mutex> lk(g_static_mu);
+  auto it = g_static_filters.find(h);
+  if (it == g_static_filters.end()) {
+    VFilter f = build_vchroma_filter(h);
+    if (f.taps < 1 || f.taps > kStaticMaxTaps || !filter_fits_tiles(f, h)) {
+      set_last_error("unsupported vertical filter size");
+      return NES_ERANGE;
+    }
+    it = g_static_filters.emplace(h, std::move(f)).first;
+  }
+  *out = &it->second;
+  return NES_OK;
+}
+
 }  // namespace nes
 
 using namespace nes;
@@ -579,6 +635,42 @@ int nes_frameprep_device(const uint8_t *rgb_a, const float *depth_a,
   }
   if (rc != NES_OK) return rc;
   NES_HIP_TRY(hipStreamSynchronize(s));
+  return NES_OK;
+}
+
+int nes_frameprep_device_static(const uint8_t *rgb_a, const float *depth_a,
+                                const uint8_t *rgb_b, const float *depth_b, int w, int h,
+                                int n_frames, uint8_t *y, int y_stride, uint8_t *u,
+                                int u_stride, uint8_t *v, int v_stride, int sws_mode,
+                                void *stream) {
+  NES_TRY(ensure_device());
+  NES_TRY(frameprep_check(rgb_a, depth_a, rgb_b, depth_b, w, h, n_frames, y, y_stride, u,
+                          u_stride, v, v_stride, sws_mode));
+  if (h > kStaticMaxHeight) {
+    set_last_error("nes_frameprep_device_static: frame height above 8192 (its fixed scratch "
+                   "holds the chroma filter of at most 4096 rows)");
+    return NES_ERANGE;
+  }
+  void *sp = nullptr;
+  NES_TRY(static_scratch(&sp));
+  const VFilter *f = nullptr;
+  NES_TRY(static_filter(h, &f));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  DevVFilter flt;
+  flt.taps = f->taps;
+  flt.coef = static_cast<int16_t *>(sp);
+  flt.pos = reinterpret_cast<int32_t *>(static_cast<uint8_t *>(sp) + kStaticCoefBytes);
+  // the table of this call's height replaces whatever the last call left
+  NES_HIP_TRY(hipMemcpyAsync(flt.coef, f->coef.data(), f->coef.size() * sizeof(int16_t),
+                             hipMemcpyHostToDevice, s));
+  NES_HIP_TRY(hipMemcpyAsync(flt.pos, f->pos.data(), f->pos.size() * sizeof(int32_t),
+                             hipMemcpyHostToDevice, s));
+  const size_t px = (size_t)w * h;
+  NES_TRY(launch_frameprep_filter(rgb_a, depth_a, rgb_b, depth_b, w, h, n_frames, 3 * px, 4 * px,
+                                  y, y_stride, (size_t)y_stride * h, u, u_stride,
+                                  (size_t)u_stride * (h / 2), v, v_stride,
+                                  (size_t)v_stride * (h / 2), sws_mode, flt, s));
+  if (!stream) NES_HIP_TRY(hipStreamSynchronize(s));
   return NES_OK;
 }
 

@@ -157,6 +157,9 @@ _sig("nes_frameprep", C.c_int, _P(C.c_uint8), _P(C.c_float), _P(C.c_uint8), _P(C
 _sig("nes_frameprep_device", C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
      C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int,
      _P(C.c_float))
+_sig("nes_frameprep_device_static", C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+     C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+     C.c_int, C.c_void_p)
 _sig("nes_gray8_to_yuv420p", C.c_int, _P(C.c_uint8), C.c_int, C.c_int, C.c_int, _P(C.c_uint8),
      C.c_int, _P(C.c_uint8), C.c_int, _P(C.c_uint8), C.c_int, C.c_int)
 _sig("nes_sws_vchroma_filter", C.c_int, C.c_int, _P(C.c_int16), _P(C.c_int32), C.c_int,
@@ -522,6 +525,17 @@ def frameprep_device(rgb_a, depth_a, rgb_b, depth_b, w, h, n_frames, y, u, v, mo
                                      n_frames, vp(y), ys, vp(u), uvs, vp(v), uvs, mode,
                                      C.byref(ms)), "nes_frameprep_device")
     return ms.value
+
+
+def frameprep_device_static(rgb_a, depth_a, rgb_b, depth_b, w, h, n_frames, y, u, v,
+                            mode=SWS_X86, y_stride=None, uv_stride=None):
+    """frameprep_device through nes_frameprep_device_static (no device
+    allocation; null stream, returns after completion)."""
+    vp = lambda b: b.ptr if b is not None else None
+    ys, uvs = y_stride or w, uv_stride or w // 2
+    check(lib().nes_frameprep_device_static(vp(rgb_a), vp(depth_a), vp(rgb_b), vp(depth_b), w, h,
+                                            n_frames, vp(y), ys, vp(u), uvs, vp(v), uvs, mode,
+                                            None), "nes_frameprep_device_static")
 
 
 def device_copy(dst, src, nbytes):
