@@ -156,7 +156,7 @@ def opaque_leg(N, views, steps=10):
     return out
 
 
-def ceilings_leg(N, samples, ms_encode, l2_req_per_launch):
+def ceilings_leg(N, samples, ms_encode, l2_req_per_launch, l1_acc_per_launch=None):
     """Measured gather ceilings (nes_device_gather_rate): random one-lane
     4-byte loads inside 16 KiB per XCD (vector-L1 hits: lane-address rate) and
     inside 2 MiB per XCD (L1 misses, L2 hits: L2 request rate), and the
@@ -166,6 +166,8 @@ def ceilings_leg(N, samples, ms_encode, l2_req_per_launch):
     lane_loads = 128 * samples / (ms_encode * 1e-3)  # 16 levels x 8 corners
     out = {"l1_lane_loads_per_s": l1, "l2_requests_per_s": l2,
            "encode_lane_loads_per_s": lane_loads, "encode_l1_frac": round(lane_loads / l1, 4)}
+    if l1_acc_per_launch:
+        out["encode_l1_accesses_frac"] = round(l1_acc_per_launch / (ms_encode * 1e-3) / l1, 4)
     if l2_req_per_launch:
         rate = l2_req_per_launch / (ms_encode * 1e-3)
         out.update({"encode_l2_requests_per_s": rate, "encode_l2_frac": round(rate / l2, 4),
@@ -384,8 +386,9 @@ def main():
         }
         if not args.skip_extra:
             result["ceilings"] = ceilings_leg(
-                N, samples, ms_enc, load_profile("encode_parts_f16_stereo1080p_l2.json",
-                                                 "l2_requests_per_launch"))
+                N, samples, ms_enc,
+                load_profile("encode_parts_f16_stereo1080p_l2.json", "l2_requests_per_launch"),
+                load_profile("encode_parts_f16_stereo1080p_l2.json", "l1_accesses_per_launch"))
             result["config1_640x480"] = config1_leg(N, table, weights, occ, ctx)
             result["opaque_stereo1080p"] = opaque_leg(N, views)
         if not args.skip_frameprep:
