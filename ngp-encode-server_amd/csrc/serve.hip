@@ -11,13 +11,14 @@
 //     latest pose matters: a newer request supersedes one still waiting for
 //     admission;
 //   * one scheduler thread forms batches of waiting requests as soon as a
-//     batch slot is free.  A slot owns one HIP stream and the device buffers
-//     of one batch; the number of slots is what the HBM budget affords, and
-//     each slot renders within its share of the budget (a batch whose samples
-//     would not fit is rendered again in bounded-memory passes instead of
-//     growing), so admission is by memory.  Batches on different slots
-//     overlap on the device, and a request arriving while others render joins
-//     the next free slot without waiting for them (continuous batching);
+//     batch slot is free.  A slot owns the device buffers of one batch and a
+//     stream for its output work; the number of slots is what the HBM budget
+//     affords, and each slot renders within its share of the budget (a batch
+//     whose samples would not fit is rendered again in bounded-memory passes
+//     instead of growing), so admission is by memory.  Renders run back to
+//     back on one render stream while earlier batches' frame prep and copies
+//     overlap them, and a request arriving while others render joins the next
+//     free slot without waiting for them (continuous batching);
 //   * each batch is one asynchronous launch sequence on the scene the context
 //     held when the batch was formed: the render (march, encode, MLPs,
 //     composite; one pass, or two when the slot's previous batch evaluated
@@ -132,7 +133,8 @@ static nes_frame_planes tight_planes(uint8_t *p, const nes_camera &c) {
 }
 
 struct Slot {
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;    // frame prep and device-to-host copies
+  hipEvent_t ev_rendered = nullptr;  // the batch's render is complete
   RenderWork work;
   DevBuf yuv;                            // device YUV420P planes of the batch
   DevBuf ovl_place;                      // text overlay placements (device)
@@ -163,6 +165,11 @@ struct nes_server {
   uint64_t next_ticket = 1, next_batch = 1;
   bool stop = false, hold = false;
   std::thread scheduler;
+  // Every batch's render runs on this one stream, so renders never overlap
+  // each other (concurrent encodes share the XCDs' L2s and run slower than
+  // back to back); a slot's frame prep and copies run on its own stream and
+  // overlap the next batch's render.
+  hipStream_t render_stream = nullptr;
   PinnedPool pinned;
   // statistics
   std::vector<double> latencies_ms, latencies_all_ms;
@@ -320,12 +327,17 @@ static void finish_batch(nes_server *s, Slot &sl, int status) {
   s->cv_done.notify_all();
 }
 
-// Launch the whole batch (render, frame prep, D2H) and its done event.
+// Launch the whole batch and its done event: the render on the server's
+// render stream, then frame prep and D2H on the slot's stream once the render
+// is complete.
 static int launch_batch(nes_server *s, Slot &sl) {
+  hipStream_t rs = s->render_stream;
   if (sl.bounded)
-    NES_TRY(render_launch_bounded(*sl.scene, sl.work, s->cfg.precision, sl.stream));
+    NES_TRY(render_launch_bounded(*sl.scene, sl.work, s->cfg.precision, rs));
   else
-    NES_TRY(render_launch_frame(*sl.scene, sl.work, s->cfg.precision, sl.stream));
+    NES_TRY(render_launch_frame(*sl.scene, sl.work, s->cfg.precision, rs));
+  NES_HIP_TRY(hipEventRecord(sl.ev_rendered, rs));
+  NES_HIP_TRY(hipStreamWaitEvent(sl.stream, sl.ev_rendered, 0));
   NES_TRY(launch_output(s, sl));
   NES_HIP_TRY(hipEventRecord(sl.ev_done, sl.stream));
   return NES_OK;
@@ -352,6 +364,7 @@ static void scheduler_loop(nes_server *s) {
         if (rc == NES_OK) rc = launch_batch(s, sl);
         if (rc == NES_OK) continue;
         s->last_error = rc;
+        hipStreamSynchronize(s->render_stream);
         hipStreamSynchronize(sl.stream);
         finish_batch(s, sl, rc);
       } else {
@@ -370,12 +383,15 @@ static void scheduler_loop(nes_server *s) {
         lk.unlock();
         sl.scene = ctx_scene(s->ctx);  // kept until the batch completes
         sl.bounded = false;
-        int rc = render_setup(sl.work, sl.views.data(), (int)sl.views.size(), sl.stream);
+        // the view upload goes with the render (the slot's previous batch is
+        // complete, so its buffers are free)
+        int rc = render_setup(sl.work, sl.views.data(), (int)sl.views.size(), s->render_stream);
         if (rc == NES_OK) rc = render_prepare_capacity(sl.work, s->cfg.precision);
         if (rc == NES_OK) rc = launch_batch(s, sl);
         sl.launched = Clock::now();
         if (rc != NES_OK) {
           s->last_error = rc;
+          hipStreamSynchronize(s->render_stream);
           hipStreamSynchronize(sl.stream);
           finish_batch(s, sl, rc);
         } else {
@@ -499,7 +515,7 @@ int nes_server_create(nes_ctx *ctx, const nes_server_config *cfg, nes_server **o
   if (cfg) s->cfg = *cfg;
   if (!s->cfg.hbm_budget_bytes) s->cfg.hbm_budget_bytes = 48ull << 30;
   if (!s->cfg.max_batch_rays) s->cfg.max_batch_rays = 1u << 24;
-  if (!s->cfg.max_inflight) s->cfg.max_inflight = 3;
+  if (!s->cfg.max_inflight) s->cfg.max_inflight = 2;
   if (!s->cfg.queue_capacity) s->cfg.queue_capacity = 256;
   if (!(s->cfg.depth_far > 0.0f)) s->cfg.depth_far = NES_DEPTH_FAR_DEFAULT;
   if (s->cfg.max_batch_rays > kMaxSlots) return NES_ERANGE;
@@ -522,11 +538,13 @@ int nes_server_create(nes_ctx *ctx, const nes_server_config *cfg, nes_server **o
   const uint64_t share = s->cfg.hbm_budget_bytes / n_slots;
   // the YUV staging and overlay buffers of a slot sit outside its render work
   const uint64_t staging = (uint64_t)s->cfg.max_batch_rays * (depth ? 4 : 2) + (2u << 20);
+  NES_HIP_TRY(hipStreamCreateWithFlags(&s->render_stream, hipStreamNonBlocking));
   for (uint32_t i = 0; i < n_slots; ++i) {
     auto sl = std::make_unique<Slot>();
     sl->work.budget = share - staging;
     NES_HIP_TRY(hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking));
     NES_HIP_TRY(hipEventCreateWithFlags(&sl->ev_done, hipEventDisableTiming));
+    NES_HIP_TRY(hipEventCreateWithFlags(&sl->ev_rendered, hipEventDisableTiming));
     s->slots.push_back(std::move(sl));
   }
   nes_server *raw = s.release();
@@ -549,14 +567,17 @@ int nes_server_destroy(nes_server *s) {
   }
   s->cv_work.notify_all();
   if (s->scheduler.joinable()) s->scheduler.join();
+  if (s->render_stream) hipStreamSynchronize(s->render_stream);
   for (auto &sl : s->slots) {
     hipStreamSynchronize(sl->stream);
     render_work_free(sl->work);
     sl->yuv.release();
     sl->ovl_place.release();
     hipEventDestroy(sl->ev_done);
+    hipEventDestroy(sl->ev_rendered);
     hipStreamDestroy(sl->stream);
   }
+  if (s->render_stream) hipStreamDestroy(s->render_stream);
   for (auto &kv : s->tickets) s->pinned.put(kv.second->yuv);
   delete s;
   return NES_OK;

@@ -94,6 +94,7 @@ bool decode_camera(const uint8_t *p, size_t n, nes_camera *cam, bool *has_fov) {
     const uint64_t key = r.varint();
     if (!r.ok) break;
     const int field = (int)(key >> 3), wt = (int)(key & 7);
+    if (field == 0 || (key >> 32)) return false;  // invalid tag, as the protobuf runtime
     if (field == 2 && wt == 0) {
       cam->is_left = r.varint() != 0;
     } else if (field == 3 && wt == 0) {
@@ -179,11 +180,15 @@ int nes_wire_decode_request(const uint8_t *msg, size_t len, float default_fov,
     const uint64_t key = r.varint();
     if (!r.ok) break;
     const int field = (int)(key >> 3), wt = (int)(key & 7);
+    if (field == 0 || (key >> 32)) return NES_EINVAL;  // invalid tag, as the protobuf runtime
     if (field == 1 && wt == 0) {
       cam->index = r.varint();
     } else if (field == 2 && wt == 2) {
       const uint64_t n = r.varint();
       if (!r.ok || (uint64_t)(r.end - r.p) < n) return NES_EINVAL;
+      // a second camera would be merged by the protobuf runtime (its matrix
+      // then has 24 floats); refused here
+      if (have_cam) return NES_EINVAL;
       const float keep_fov = cam->fov_y_deg;
       const uint64_t idx = cam->index;
       if (!decode_camera(r.p, n, cam, &has_fov)) return NES_EINVAL;
