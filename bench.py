@@ -234,11 +234,22 @@ def frameprep_leg(N, n_frames=256, w=1280, h=720):
     ms = sorted(N.frameprep_device(*bufs, w, h, n_frames, Y, U, V) for _ in range(10))[5]
     for buf in bufs + [Y, U, V]:
         buf.free()
+    # the practical streaming ceiling: a plain 1 GiB device-to-device copy
+    # (read + write), the same byte mix a pure streaming kernel moves
+    nbytes = 1 << 30
+    src, dst = N.DeviceBuffer(nbytes), N.DeviceBuffer(nbytes)
+    N.device_copy(dst, src, nbytes)
+    copy_ms = sorted(N.device_copy(dst, src, nbytes) for _ in range(5))[2]
+    src.free()
+    dst.free()
+    copy_gbs = 2 * nbytes / copy_ms / 1e6
     bytes_per_frame = w * h * 15.5
+    gbs = bytes_per_frame * n_frames / ms / 1e6
     return {"frames_per_launch": n_frames, "ms_per_launch": ms,
             "frames_per_s": n_frames / ms * 1e3,
-            "achieved_GBs": bytes_per_frame * n_frames / ms / 1e6,
-            "hbm_frac": bytes_per_frame * n_frames / ms / 1e6 / PEAK_HBM_GBS}
+            "achieved_GBs": gbs, "hbm_frac": gbs / PEAK_HBM_GBS,
+            "copy_ceiling_GBs": round(copy_gbs, 1),
+            "frac_of_copy_ceiling": round(gbs / copy_gbs, 4)}
 
 
 def training_leg(N, ctx, occ, steps=12, warmup=3):
