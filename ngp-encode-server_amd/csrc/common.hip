@@ -72,6 +72,24 @@ void DevBuf::release() {
   bytes = 0;
 }
 
+int table_copy(void *dst, bool dst_dev, const void *src, bool src_dev, size_t feat_bytes,
+               hipMemcpyKind kind, hipStream_t stream) {
+  const size_t entry = 2 * feat_bytes;
+  for (int l = 0; l < kLevels; ++l) {
+    const size_t n = (size_t)(kLevelOffset[l + 1] - kLevelOffset[l]) * entry;
+    char *d = static_cast<char *>(dst) + (size_t)(dst_dev ? kDevLevelOffset[l] : kLevelOffset[l]) * entry;
+    const char *s =
+        static_cast<const char *>(src) + (size_t)(src_dev ? kDevLevelOffset[l] : kLevelOffset[l]) * entry;
+    if (kind == hipMemcpyHostToHost)
+      std::memcpy(d, s, n);
+    else if (stream)
+      NES_HIP_TRY(hipMemcpyAsync(d, s, n, kind, stream));
+    else
+      NES_HIP_TRY(hipMemcpy(d, s, n, kind));
+  }
+  return NES_OK;
+}
+
 // Gather-rate probe: random, independent 4-byte loads (one address per lane)
 // within a per-XCD region of the buffer -- workgroup b runs on XCD b % 8 and
 // reads region b % 8.  A region that fits the vector L1 measures the lane

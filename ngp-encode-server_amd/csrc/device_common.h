@@ -190,7 +190,7 @@ __device__ __forceinline__ PartLevels part_levels(int part) {
     for (int c = 0; c < 16; ++c)
       if (l == c) {
         res = kLevelRes[c];
-        off = kLevelOffset[c];
+        off = kDevLevelOffset[c];
       }
     p.scale[q] = (float)res;
     p.n1[q] = (uint32_t)res + 1u;
@@ -202,14 +202,15 @@ __device__ __forceinline__ PartLevels part_levels(int part) {
 }
 
 // Compile-time constants of level L for table type Tab (FP16: 4-byte
-// entries, FP32: 8-byte entries; both in the reference layout).
+// entries, FP32: 8-byte entries; both in the device layout, each level
+// 32-entry aligned).
 template <class Tab, int L>
 struct LevelC {
   static constexpr bool kF16 = std::is_same<Tab, _Float16>::value;
   static constexpr float kScale = (float)kLevelRes[L];
   static constexpr uint32_t kN1 = (uint32_t)kLevelRes[L] + 1u;
   static constexpr bool kDense = L < kDenseLevels;
-  static constexpr uint32_t kOff = kLevelOffset[L];
+  static constexpr uint32_t kOff = kDevLevelOffset[L];
   static constexpr int kShift = kF16 ? 2 : 3;  // log2 bytes per entry
 };
 

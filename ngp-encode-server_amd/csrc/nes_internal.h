@@ -55,6 +55,37 @@ constexpr uint32_t kLevelOffset[kLevels + 1] = {
     856045u,  1380333u, 1904621u, 2428909u, 2953197u, 3477485u,
     4001773u, 4526061u, 5050349u, 5574637u, 6098925u};
 
+// Device layout of the table: the same levels with each one starting on a
+// 32-entry boundary (128 B of FP16 entries, 256 B of FP32).  Packed level
+// after level, every hashed level starts 5 entries past a 64 B line, so the
+// pair of x corners (hash indices h and h ^ (x ^ (x + 1))) straddles two
+// lines 22 % of the time instead of 6 % (FP32 scatter: 5.5 requests per
+// sample and level instead of 4.5).  Host buffers keep the packed layout;
+// every copy between the two goes level by level (table_copy).
+constexpr uint32_t align32(uint32_t x) { return (x + 31u) & ~31u; }
+constexpr uint32_t dev_level_offset(int l) {
+  return l == 0 ? 0u
+                : align32(dev_level_offset(l - 1) + (kLevelOffset[l] - kLevelOffset[l - 1]));
+}
+constexpr uint32_t kDevLevelOffset[kLevels + 1] = {
+    dev_level_offset(0),  dev_level_offset(1),  dev_level_offset(2),  dev_level_offset(3),
+    dev_level_offset(4),  dev_level_offset(5),  dev_level_offset(6),  dev_level_offset(7),
+    dev_level_offset(8),  dev_level_offset(9),  dev_level_offset(10), dev_level_offset(11),
+    dev_level_offset(12), dev_level_offset(13), dev_level_offset(14), dev_level_offset(15),
+    dev_level_offset(16)};
+constexpr uint32_t kDevTableEntries = kDevLevelOffset[kLevels];
+static_assert(kDevTableEntries == 6099008u, "device table layout");
+
+// Copy a table of 2 features per entry (feat_bytes each: 4 FP32, 2 FP16)
+// between the packed layout and the device layout, level by level.
+// dst_dev / src_dev say which side is in the device layout.  With a null
+// stream every level is a synchronous hipMemcpy (host memory may be
+// released on return); otherwise the copies are queued on `stream` and the
+// caller keeps host memory alive until the stream has passed them.  Host to
+// host copies use memcpy.
+int table_copy(void *dst, bool dst_dev, const void *src, bool src_dev, size_t feat_bytes,
+               hipMemcpyKind kind, hipStream_t stream);
+
 // MLP layout (row-major [out][in] per layer).
 constexpr int kW1 = 0, kW2 = 2048, kW3 = 3072, kW4 = 5056, kW5 = 9152;
 

@@ -599,20 +599,23 @@ void build_mlp_fragments(const float *w, _Float16 *frag) {
 
 int scene_upload(SceneDev &s, const nes_scene_desc *desc) {
   if (!desc || !desc->table || !desc->weights || !desc->occupancy) return NES_EINVAL;
-  const size_t nt = (size_t)kTableEntries * 2;
+  const size_t nt = (size_t)kTableEntries * 2, ndev = (size_t)kDevTableEntries * 2;
   if (!s.table_f32) {
-    NES_HIP_TRY(hipMalloc(&s.table_f32, nt * sizeof(float)));
-    NES_HIP_TRY(hipMalloc(&s.table_f16, nt * sizeof(_Float16)));
+    NES_HIP_TRY(hipMalloc(&s.table_f32, ndev * sizeof(float)));
+    NES_HIP_TRY(hipMalloc(&s.table_f16, ndev * sizeof(_Float16)));
     NES_HIP_TRY(hipMalloc(&s.weights_f32, kWeights * sizeof(float)));
     NES_HIP_TRY(hipMalloc(&s.wfrag, 20 * 64 * 8 * sizeof(_Float16)));
     NES_HIP_TRY(hipMalloc(&s.occ, NES_OCC_WORDS * sizeof(uint32_t)));
+    // the padding between levels (device layout) is never read; keep it zero
+    NES_HIP_TRY(hipMemset(s.table_f32, 0, ndev * sizeof(float)));
+    NES_HIP_TRY(hipMemset(s.table_f16, 0, ndev * sizeof(_Float16)));
   }
   std::vector<_Float16> t16(nt);
   for (size_t i = 0; i < nt; ++i) t16[i] = (_Float16)desc->table[i];
   std::vector<_Float16> frag(20 * 64 * 8);
   build_mlp_fragments(desc->weights, frag.data());
-  NES_HIP_TRY(hipMemcpy(s.table_f32, desc->table, nt * sizeof(float), hipMemcpyHostToDevice));
-  NES_HIP_TRY(hipMemcpy(s.table_f16, t16.data(), nt * sizeof(_Float16), hipMemcpyHostToDevice));
+  NES_TRY(table_copy(s.table_f32, true, desc->table, false, sizeof(float), hipMemcpyHostToDevice, nullptr));
+  NES_TRY(table_copy(s.table_f16, true, t16.data(), false, sizeof(_Float16), hipMemcpyHostToDevice, nullptr));
   NES_HIP_TRY(hipMemcpy(s.weights_f32, desc->weights, kWeights * sizeof(float), hipMemcpyHostToDevice));
   NES_HIP_TRY(hipMemcpy(s.wfrag, frag.data(), frag.size() * sizeof(_Float16), hipMemcpyHostToDevice));
   NES_HIP_TRY(hipMemcpy(s.occ, desc->occupancy, NES_OCC_WORDS * sizeof(uint32_t), hipMemcpyHostToDevice));

@@ -504,7 +504,7 @@ __global__ __launch_bounds__(256) void scatter_parts(const float4 *__restrict__ 
   const uint32_t xb = (threadIdx.x >> 1) & 1u, ft = threadIdx.x & 1u;
   const bool in_lds = part <= 1;  // block-uniform: level 0 (part 0) or 1 (part 1) in LDS
   const uint32_t nfl = part == 0 ? kLevel0Floats : kLevel1Floats;
-  const uint32_t off = part == 0 ? 0u : 2u * kLevelOffset[1];
+  const uint32_t off = part == 0 ? 0u : 2u * kDevLevelOffset[1];
   if (in_lds)
     for (uint32_t i = threadIdx.x; i < nfl; i += blockDim.x) dense_sum[i] = 0.0f;
   __syncthreads();
@@ -577,7 +577,7 @@ using namespace nes;
 
 struct nes_trainer {
   nes_train_config cfg{};
-  size_t n_table = (size_t)kTableEntries * 2, n_params = 0;
+  size_t n_table = (size_t)kDevTableEntries * 2, n_params = 0;  // device layout
   float *params = nullptr, *grads = nullptr, *m = nullptr, *v = nullptr;
   uint32_t *occ = nullptr;
   ViewDesc *poses = nullptr;
@@ -737,7 +737,9 @@ int nes_trainer_create(const nes_scene_desc *init, const nes_train_config *cfg,
   NES_HIP_TRY(hipMalloc(&t->target, nr * 3 * sizeof(float)));
   NES_TRY(t->dCb.reserve(nr * 3 * sizeof(float)));
   t->dC = t->dCb.as<float>();
-  NES_HIP_TRY(hipMemcpy(t->params, init->table, t->n_table * sizeof(float), hipMemcpyHostToDevice));
+  // the padding between levels stays zero (gradients and moments there stay 0)
+  NES_HIP_TRY(hipMemset(t->params, 0, t->n_params * sizeof(float)));
+  NES_TRY(table_copy(t->params, true, init->table, false, sizeof(float), hipMemcpyHostToDevice, nullptr));
   NES_HIP_TRY(hipMemcpy(t->params + t->n_table, init->weights, kWeights * sizeof(float),
                         hipMemcpyHostToDevice));
   NES_HIP_TRY(hipMemcpy(t->occ, init->occupancy, NES_OCC_WORDS * sizeof(uint32_t),
@@ -822,7 +824,7 @@ int nes_trainer_batch_grad(nes_trainer *t, uint64_t step, float *loss, float *gr
   zero_f32<<<1024, 256, 0, s>>>(t->grads, t->n_params);
   NES_TRY(trainer_batch(t, step, loss));
   if (grad_table)
-    NES_HIP_TRY(hipMemcpyAsync(grad_table, t->grads, t->n_table * sizeof(float), hipMemcpyDeviceToHost, s));
+    NES_TRY(table_copy(grad_table, false, t->grads, true, sizeof(float), hipMemcpyDeviceToHost, s));
   if (grad_weights)
     NES_HIP_TRY(hipMemcpyAsync(grad_weights, t->grads + t->n_table, kWeights * sizeof(float),
                                hipMemcpyDeviceToHost, s));
@@ -835,7 +837,7 @@ int nes_trainer_get_params(nes_trainer *t, float *table, float *weights) {
   if (!t) return NES_EINVAL;
   std::lock_guard<std::mutex> lk(t->mu);
   NES_HIP_TRY(hipStreamSynchronize(t->stream));
-  if (table) NES_HIP_TRY(hipMemcpy(table, t->params, t->n_table * sizeof(float), hipMemcpyDeviceToHost));
+  if (table) NES_TRY(table_copy(table, false, t->params, true, sizeof(float), hipMemcpyDeviceToHost, nullptr));
   if (weights)
     NES_HIP_TRY(hipMemcpy(weights, t->params + t->n_table, kWeights * sizeof(float), hipMemcpyDeviceToHost));
   return NES_OK;
@@ -845,7 +847,7 @@ int nes_trainer_set_params(nes_trainer *t, const float *table, const float *weig
   if (!t) return NES_EINVAL;
   std::lock_guard<std::mutex> lk(t->mu);
   NES_HIP_TRY(hipStreamSynchronize(t->stream));
-  if (table) NES_HIP_TRY(hipMemcpy(t->params, table, t->n_table * sizeof(float), hipMemcpyHostToDevice));
+  if (table) NES_TRY(table_copy(t->params, true, table, false, sizeof(float), hipMemcpyHostToDevice, nullptr));
   if (weights)
     NES_HIP_TRY(hipMemcpy(t->params + t->n_table, weights, kWeights * sizeof(float), hipMemcpyHostToDevice));
   return NES_OK;
@@ -901,7 +903,7 @@ int nes_trainer_backward(nes_trainer *t, const float *ray_o, const float *ray_d,
   zero_f32<<<1024, 256, 0, s>>>(t->grads, t->n_params);
   NES_TRY(trainer_backward(t, n));
   if (grad_table)
-    NES_HIP_TRY(hipMemcpyAsync(grad_table, t->grads, t->n_table * sizeof(float), hipMemcpyDeviceToHost, s));
+    NES_TRY(table_copy(grad_table, false, t->grads, true, sizeof(float), hipMemcpyDeviceToHost, s));
   if (grad_weights)
     NES_HIP_TRY(hipMemcpyAsync(grad_weights, t->grads + t->n_table, kWeights * sizeof(float),
                                hipMemcpyDeviceToHost, s));
@@ -913,7 +915,8 @@ int nes_trainer_backward(nes_trainer *t, const float *ray_o, const float *ray_d,
 // ------------------------------------------------------------ checkpoint
 // File layout (little-endian): "NESCKPT1", u32 version 1, u32 weights,
 // u64 table floats, u64 step, then f32 parameters (table, weights), first
-// moments and second moments, n_params each.
+// moments and second moments, table floats + weights each; the table parts
+// in the packed layout of the C-ABI (level after level, no padding).
 namespace {
 const char kCkptMagic[8] = {'N', 'E', 'S', 'C', 'K', 'P', 'T', '1'};
 struct CkptHeader {
@@ -927,7 +930,8 @@ int nes_trainer_save(nes_trainer *t, const char *path) {
   if (!t || !path) return NES_EINVAL;
   std::lock_guard<std::mutex> lk(t->mu);
   NES_HIP_TRY(hipStreamSynchronize(t->stream));
-  std::vector<float> buf(t->n_params);
+  const size_t n_packed = (size_t)kTableEntries * 2 + kWeights;
+  std::vector<float> dev(t->n_params), packed(n_packed);
   FILE *f = std::fopen(path, "wb");
   if (!f) {
     set_last_error(std::string("cannot create ") + path);
@@ -937,16 +941,18 @@ int nes_trainer_save(nes_trainer *t, const char *path) {
   std::memcpy(h.magic, kCkptMagic, 8);
   h.version = 1;
   h.weights = kWeights;
-  h.table = t->n_table;
+  h.table = (uint64_t)kTableEntries * 2;
   h.step = t->step;
   bool ok = std::fwrite(&h, sizeof(h), 1, f) == 1;
   for (float *src : {t->params, t->m, t->v}) {
     if (!ok) break;
-    if (hipMemcpy(buf.data(), src, t->n_params * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) {
+    if (hipMemcpy(dev.data(), src, t->n_params * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) {
       ok = false;
       break;
     }
-    ok = std::fwrite(buf.data(), sizeof(float), t->n_params, f) == t->n_params;
+    table_copy(packed.data(), false, dev.data(), true, sizeof(float), hipMemcpyHostToHost, nullptr);
+    std::memcpy(packed.data() + h.table, dev.data() + t->n_table, kWeights * sizeof(float));
+    ok = std::fwrite(packed.data(), sizeof(float), n_packed, f) == n_packed;
   }
   ok = (std::fclose(f) == 0) && ok;
   if (!ok) {
@@ -966,24 +972,29 @@ int nes_trainer_load(nes_trainer *t, const char *path) {
     return NES_EIO;
   }
   CkptHeader h;
+  const size_t n_packed = (size_t)kTableEntries * 2 + kWeights;
   if (std::fread(&h, sizeof(h), 1, f) != 1 || std::memcmp(h.magic, kCkptMagic, 8) != 0 ||
-      h.version != 1 || h.weights != (uint32_t)kWeights || h.table != t->n_table) {
+      h.version != 1 || h.weights != (uint32_t)kWeights || h.table != (uint64_t)kTableEntries * 2) {
     std::fclose(f);
     set_last_error("not a checkpoint of this scene representation");
     return NES_EINVAL;
   }
   // read everything before touching the trainer, so a short file changes nothing
-  std::vector<float> buf(3 * t->n_params);
+  std::vector<float> buf(3 * n_packed);
   const bool ok = std::fread(buf.data(), sizeof(float), buf.size(), f) == buf.size();
   std::fclose(f);
   if (!ok) {
     set_last_error("checkpoint file is truncated");
     return NES_EINVAL;
   }
+  std::vector<float> dev(t->n_params, 0.0f);  // level padding stays zero
   float *dst[3] = {t->params, t->m, t->v};
-  for (int i = 0; i < 3; ++i)
-    NES_HIP_TRY(hipMemcpy(dst[i], buf.data() + (size_t)i * t->n_params, t->n_params * sizeof(float),
-                          hipMemcpyHostToDevice));
+  for (int i = 0; i < 3; ++i) {
+    const float *src = buf.data() + (size_t)i * n_packed;
+    table_copy(dev.data(), true, src, false, sizeof(float), hipMemcpyHostToHost, nullptr);
+    std::memcpy(dev.data() + t->n_table, src + h.table, kWeights * sizeof(float));
+    NES_HIP_TRY(hipMemcpy(dst[i], dev.data(), t->n_params * sizeof(float), hipMemcpyHostToDevice));
+  }
   t->step = h.step;
   return NES_OK;
 }
