@@ -346,8 +346,9 @@ typedef struct nes_server nes_server;
 typedef struct nes_server_config {
   uint64_t hbm_budget_bytes; /* 0 = 48 GiB */
   uint32_t max_batch_rays;   /* ray slots per batch, 0 = 2^24 */
-  uint32_t max_inflight;     /* batch slots, 0 = 2 (renders run one at a time; a
-                                second slot overlaps the first one's output) */
+  uint32_t max_inflight;     /* batch slots, 0 = 3 (renders run one at a time,
+                                at most two queued; the other slots hold earlier
+                                batches' frame prep and copies) */
   uint32_t queue_capacity;   /* waiting requests, 0 = 256 */
   int precision;             /* NES_PRECISION_* */
   int sws_mode;              /* NES_SWS_* */

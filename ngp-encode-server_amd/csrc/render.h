@@ -34,6 +34,8 @@ struct RayAcc {  // 32 bytes, compositing state carried across passes
 
 // Sample record: ray slot in the high 25 bits, step index in the low 7.
 constexpr uint32_t kMaxSlots = 1u << 25;
+// View descriptors the pinned upload staging holds before it grows.
+constexpr uint32_t kPinnedViews = 64;
 
 // Per ray slot, the march writes counts[slot] = samples emitted by the pass in
 // the low 16 bits and the step the next pass resumes at in the high 16 bits.
@@ -68,6 +70,12 @@ struct RenderWork {
   int capacity_precision = -1;    // feature layout the buffers were sized for
   size_t out_px = 0;
   std::vector<ViewDesc> host_views;
+  // Pinned staging of the view descriptors: an upload from pageable memory
+  // can block the host until the stream reaches it.  ev_views marks the
+  // last upload, so the staging is rewritten only after it has been read.
+  ViewDesc *views_pinned = nullptr;
+  uint32_t views_pinned_cap = 0;
+  hipEvent_t ev_views = nullptr;
   uint64_t budget = 0;            // device bytes the buffers may hold, 0 = no limit
   // Early termination learned from the previous frame of this work area:
   // samples the first pass gives each ray (kMaxSteps = one single pass).
@@ -84,6 +92,11 @@ struct FrameCounts {
 
 // Device bytes currently held by a RenderWork's buffers.
 size_t work_device_bytes(const RenderWork &w);
+// Reserve every buffer a render of up to max_slots ray slots, max_px pixels
+// and max_views views needs (with the initial four samples per ray slot),
+// so that such renders allocate nothing.
+int render_work_reserve(RenderWork &w, uint64_t max_slots, uint64_t max_px, uint32_t max_views,
+                        int precision);
 
 struct SceneDev {
   float *table_f32 = nullptr;      // entries x 2

@@ -637,6 +637,14 @@ void render_work_free(RenderWork &w) {
     b->release();
   if (w.total_host) hipHostFree(w.total_host);
   w.total_host = nullptr;
+  if (w.ev_views) {
+    hipEventSynchronize(w.ev_views);
+    hipEventDestroy(w.ev_views);
+    w.ev_views = nullptr;
+  }
+  if (w.views_pinned) hipHostFree(w.views_pinned);
+  w.views_pinned = nullptr;
+  w.views_pinned_cap = 0;
   for (hipEvent_t &e : w.ev)
     if (e) {
       hipEventDestroy(e);
@@ -715,9 +723,36 @@ int render_setup(RenderWork &w, const nes_camera *views, int n_views, hipStream_
   if (!w.total_host) NES_HIP_TRY(hipHostMalloc((void **)&w.total_host, kTotBytes));
   for (hipEvent_t &e : w.ev)
     if (!e) NES_HIP_TRY(hipEventCreate(&e));
-  NES_HIP_TRY(hipMemcpyAsync(w.views.ptr, w.host_views.data(), sizeof(ViewDesc) * n_views,
+  if (!w.ev_views) NES_HIP_TRY(hipEventCreateWithFlags(&w.ev_views, hipEventDisableTiming));
+  NES_HIP_TRY(hipEventSynchronize(w.ev_views));  // the previous upload has read the staging
+  if ((uint32_t)n_views > w.views_pinned_cap) {
+    if (w.views_pinned) NES_HIP_TRY(hipHostFree(w.views_pinned));
+    w.views_pinned = nullptr;
+    w.views_pinned_cap = 0;
+    const uint32_t cap = std::max<uint32_t>((uint32_t)n_views, kPinnedViews);
+    NES_HIP_TRY(hipHostMalloc((void **)&w.views_pinned, sizeof(ViewDesc) * cap));
+    w.views_pinned_cap = cap;
+  }
+  std::memcpy(w.views_pinned, w.host_views.data(), sizeof(ViewDesc) * n_views);
+  NES_HIP_TRY(hipMemcpyAsync(w.views.ptr, w.views_pinned, sizeof(ViewDesc) * n_views,
                              hipMemcpyHostToDevice, stream));
+  NES_HIP_TRY(hipEventRecord(w.ev_views, stream));
   return NES_OK;
+}
+
+int render_work_reserve(RenderWork &w, uint64_t max_slots, uint64_t max_px, uint32_t max_views,
+                        int precision) {
+  if (max_slots > kMaxSlots) return NES_ERANGE;
+  NES_TRY(w.views.reserve(sizeof(ViewDesc) * std::max<uint32_t>(max_views, kPinnedViews)));
+  NES_TRY(w.rays.reserve(sizeof(RayData) * max_slots));
+  NES_TRY(w.acc.reserve(sizeof(RayAcc) * max_slots));
+  NES_TRY(w.counts.reserve(sizeof(uint32_t) * max_slots));
+  NES_TRY(w.offsets.reserve(sizeof(uint32_t) * max_slots));
+  NES_TRY(w.ray_sh.reserve(32 * max_slots));
+  NES_TRY(w.tinfo.reserve(8 * max_slots));
+  NES_TRY(w.out.reserve(out_layout(max_px).total));
+  NES_TRY(w.total.reserve(kTotBytes));
+  return render_reserve_samples(w, precision, std::max<size_t>(max_slots * 4, (size_t)1 << 20));
 }
 
 int render_reserve_samples(RenderWork &w, int precision, size_t capacity) {

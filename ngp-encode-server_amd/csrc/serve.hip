@@ -11,14 +11,17 @@
 //     latest pose matters: a newer request supersedes one still waiting for
 //     admission;
 //   * one scheduler thread forms batches of waiting requests as soon as a
-//     batch slot is free.  A slot owns the device buffers of one batch and a
-//     stream for its output work; the number of slots is what the HBM budget
-//     affords, and each slot renders within its share of the budget (a batch
-//     whose samples would not fit is rendered again in bounded-memory passes
-//     instead of growing), so admission is by memory.  Renders run back to
-//     back on one render stream while earlier batches' frame prep and copies
-//     overlap them, and a request arriving while others render joins the next
-//     free slot without waiting for them (continuous batching);
+//     batch slot is free and fewer than two batches are queued or running on
+//     the render stream.  A slot owns the device buffers of one batch
+//     (allocated for the largest batch when the server starts, so no render
+//     allocates) and a stream for its output work; the number of slots is
+//     what the HBM budget affords, and each slot renders within its share of
+//     the budget (a batch whose samples would not fit is rendered again in
+//     bounded-memory passes instead of growing), so admission is by memory.
+//     Renders run back to back on one render stream while earlier batches'
+//     frame prep and copies overlap them, and a request arriving while others
+//     render joins the next batch without waiting for them (continuous
+//     batching);
 //   * each batch is one asynchronous launch sequence on the scene the context
 //     held when the batch was formed: the render (march, encode, MLPs,
 //     composite; one pass, or two when the slot's previous batch evaluated
@@ -53,6 +56,8 @@ static double ms_between(Clock::time_point a, Clock::time_point b) {
 
 constexpr int kMaxViews = 2;
 constexpr int NES_ECANCELED_ = NES_ECANCELED;
+// Batches queued or running on the render stream at once (see scheduler_loop).
+constexpr int kRenderDepth = 2;
 
 struct PinnedBlock {
   uint8_t *ptr = nullptr;
@@ -235,6 +240,10 @@ static int launch_output(nes_server *s, Slot &sl) {
     NES_TRY(overlay_launch(s->cfg.font, sl.ovl_place, sl.ovl_host, w.out.as<uint8_t>(),
                            sl.stream));
   }
+  // Frame prep of every view first, then every copy: each change between a
+  // kernel and a copy on the slot stream is a hand-over between the
+  // dispatcher and the copy engine that costs tens of microseconds, so the
+  // batch makes it once instead of once per plane.
   const float scale = depth_scale(s->cfg.depth_far);
   size_t off = 0;
   size_t vi = 0;
@@ -248,7 +257,6 @@ static int launch_output(nes_server *s, Slot &sl) {
       NES_TRY(launch_frameprep(rgb, nullptr, nullptr, nullptr, cw, ch, 1, 0, 0, d.data[0], cw, 0,
                                d.data[1], cw / 2, 0, d.data[2], cw / 2, 0, s->cfg.sws_mode,
                                sl.stream));
-      NES_TRY(copy_planes(rq->planes[v], d.data[0], c, sl.stream));
       off += yuv_bytes(c);
       if (depth) {
         const float *dep = reinterpret_cast<const float *>(w.out.as<uint8_t>() + L.depth) +
@@ -256,10 +264,32 @@ static int launch_output(nes_server *s, Slot &sl) {
         const nes_frame_planes dd = tight_planes(sl.yuv.as<uint8_t>() + off, c);
         NES_TRY(launch_depth_yuv(dep, cw, ch, scale, nullptr, dd.data[0], cw, dd.data[1], cw / 2,
                                  dd.data[2], cw / 2, s->cfg.sws_mode, sl.stream));
-        NES_TRY(copy_planes(rq->dplanes[v], dd.data[0], c, sl.stream));
         off += yuv_bytes(c);
       }
     }
+  }
+  // The server's own planes of a request are laid out exactly as its part
+  // of the staging (scene, then depth, per view): one copy per request.
+  off = 0;
+  for (auto &rq : sl.reqs) {
+    size_t bytes = 0;
+    for (int v = 0; v < rq->n_views; ++v) bytes += yuv_bytes(rq->views[v]) * (depth ? 2 : 1);
+    if (!rq->caller) {
+      NES_HIP_TRY(hipMemcpyAsync(rq->yuv.ptr, sl.yuv.as<uint8_t>() + off, bytes,
+                                 hipMemcpyDeviceToHost, sl.stream));
+    } else {
+      size_t o = off;
+      for (int v = 0; v < rq->n_views; ++v) {
+        const nes_camera &c = rq->views[v];
+        NES_TRY(copy_planes(rq->planes[v], sl.yuv.as<uint8_t>() + o, c, sl.stream));
+        o += yuv_bytes(c);
+        if (depth) {
+          NES_TRY(copy_planes(rq->dplanes[v], sl.yuv.as<uint8_t>() + o, c, sl.stream));
+          o += yuv_bytes(c);
+        }
+      }
+    }
+    off += bytes;
   }
   return NES_OK;
 }
@@ -371,11 +401,19 @@ static void scheduler_loop(nes_server *s) {
         finish_batch(s, sl, NES_OK);
       }
     }
-    // admit waiting requests into free slots
+    // admit waiting requests into free slots, keeping at most kRenderDepth
+    // batches queued or running on the render stream: one renders while the
+    // next waits, so the stream never drains while a slot is busy with its
+    // output, and requests that arrive meanwhile join one batch instead of
+    // being spread over many small ones
     {
+      int rendering = 0;
+      for (auto &slp : s->slots)
+        if (slp->stage == 2 && hipEventQuery(slp->ev_rendered) != hipSuccess) ++rendering;
       std::unique_lock<std::mutex> lk(s->mu);
       for (auto &slp : s->slots) {
         Slot &sl = *slp;
+        if (rendering >= kRenderDepth) break;
         if (sl.stage != 0 || s->waiting.empty() || s->hold) continue;
         form_batch(s, sl);
         if (sl.reqs.empty()) continue;
@@ -396,6 +434,7 @@ static void scheduler_loop(nes_server *s) {
           finish_batch(s, sl, rc);
         } else {
           sl.stage = 2;
+          ++rendering;
         }
         lk.lock();
         progressed = true;
@@ -515,7 +554,7 @@ int nes_server_create(nes_ctx *ctx, const nes_server_config *cfg, nes_server **o
   if (cfg) s->cfg = *cfg;
   if (!s->cfg.hbm_budget_bytes) s->cfg.hbm_budget_bytes = 48ull << 30;
   if (!s->cfg.max_batch_rays) s->cfg.max_batch_rays = 1u << 24;
-  if (!s->cfg.max_inflight) s->cfg.max_inflight = 2;
+  if (!s->cfg.max_inflight) s->cfg.max_inflight = 3;
   if (!s->cfg.queue_capacity) s->cfg.queue_capacity = 256;
   if (!(s->cfg.depth_far > 0.0f)) s->cfg.depth_far = NES_DEPTH_FAR_DEFAULT;
   if (s->cfg.max_batch_rays > kMaxSlots) return NES_ERANGE;
@@ -542,6 +581,15 @@ int nes_server_create(nes_ctx *ctx, const nes_server_config *cfg, nes_server **o
   for (uint32_t i = 0; i < n_slots; ++i) {
     auto sl = std::make_unique<Slot>();
     sl->work.budget = share - staging;
+    // Allocate the slot's buffers for the largest batch now: growing one
+    // later frees the old buffer, and hipFree waits for the whole device,
+    // draining the render stream while the scheduler thread is blocked.
+    if (per_slot <= sl->work.budget) {
+      NES_TRY(render_work_reserve(sl->work, s->cfg.max_batch_rays, s->cfg.max_batch_rays,
+                                  kPinnedViews, s->cfg.precision));
+      // YUV420P is 1.5 bytes per pixel, twice with the depth stream
+      NES_TRY(sl->yuv.reserve((size_t)s->cfg.max_batch_rays * 3 / 2 * (depth ? 2 : 1) + 256));
+    }
     NES_HIP_TRY(hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking));
     NES_HIP_TRY(hipEventCreateWithFlags(&sl->ev_done, hipEventDisableTiming));
     NES_HIP_TRY(hipEventCreateWithFlags(&sl->ev_rendered, hipEventDisableTiming));
