@@ -255,6 +255,86 @@ __device__ __forceinline__ typename RawFeat<Tab>::type load_entry(const char *le
   return __ldg(reinterpret_cast<const typename RawFeat<Tab>::type *>(level_base + boff));
 }
 
+// Entries e and e + 1 with one load (8 B FP16 / 16 B FP32 at the alignment
+// of one entry: gfx950 serves unaligned multi-dword loads).
+typedef uint2 __attribute__((aligned(4))) EntryPairF16;
+typedef float4 __attribute__((aligned(8))) EntryPairF32;
+__device__ __forceinline__ void load_pair(const char *level_base, uint32_t boff, uint32_t &a,
+                                          uint32_t &b) {
+  const EntryPairF16 p = *reinterpret_cast<const EntryPairF16 *>(level_base + boff);
+  a = p.x;
+  b = p.y;
+}
+__device__ __forceinline__ void load_pair(const char *level_base, uint32_t boff, float2 &a,
+                                          float2 &b) {
+  const EntryPairF32 p = *reinterpret_cast<const EntryPairF32 *>(level_base + boff);
+  a = make_float2(p.x, p.y);
+  b = make_float2(p.z, p.w);
+}
+
+__device__ __forceinline__ uint32_t pick4(const uint4 &q, uint32_t i) {
+  const uint32_t lo = (i & 1u) ? q.y : q.x, hi = (i & 1u) ? q.w : q.z;
+  return (i & 2u) ? hi : lo;
+}
+
+// The 8 corner entries of level L.  The gathers are bound by the vector
+// L1's address rate: one address per lane and instruction, whatever the
+// width up to 16 B, and a lane masked off still takes its slot.
+// * Dense levels: corners c and c + 1 are the adjacent entries e and e + 1,
+//   one load per pair.
+// * Hashed FP16 levels: corners c and c + 1 of a (y, z) pair are entries h
+//   and h ^ (x ^ (x + 1)), in one aligned 16 B chunk (the level starts on a
+//   128 B boundary) unless x % 4 == 3 -- the same for all four pairs of a
+//   lane.  One chunk load per pair serves both corners; the second corners
+//   of the lanes with x % 4 == 3 (a quarter of them, four loads each) are
+//   loaded by the whole wavefront, packed into as few instructions as their
+//   count needs, through the wavefront's 256-word LDS exchange `xch`.  The
+//   finest levels (kChunkedMaxLevel and up) keep one gather per corner:
+//   there almost every corner misses L1, and the 16 B loads made the XCDs
+//   encoding them slower (measured per XCD).
+constexpr int kChunkedMaxLevel = 13;
+template <class Tab, int L>
+__device__ __forceinline__ void load_corners(const char *base, const uint32_t boff[8],
+                                             typename RawFeat<Tab>::type v[8], uint32_t *xch) {
+  if constexpr (LevelC<Tab, L>::kDense) {
+#pragma unroll
+    for (int c = 0; c < 8; c += 2) load_pair(base, boff[c], v[c], v[c + 1]);
+  } else if constexpr (LevelC<Tab, L>::kF16 && L <= kChunkedMaxLevel) {
+    const bool same = ((boff[0] ^ boff[1]) >> 4) == 0u;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      const uint32_t b0 = boff[2 * p], b1 = boff[2 * p + 1];
+      const uint4 q = __ldg(reinterpret_cast<const uint4 *>(base + (b0 & ~15u)));
+      v[2 * p] = pick4(q, (b0 >> 2) & 3u);
+      v[2 * p + 1] = pick4(q, (b1 >> 2) & 3u);
+    }
+    const uint64_t needy = __ballot(!same);
+    if (needy) {  // wavefront-uniform
+      const int lane = lane_id();
+      const uint64_t below = lanes_below(lane);
+      const uint64_t act = __ballot(true);
+      const uint32_t r = (uint32_t)__popcll(needy & below);
+      const uint32_t slots = 4u * (uint32_t)__popcll(needy);
+      const uint32_t me = (uint32_t)__popcll(act & below), n_act = (uint32_t)__popcll(act);
+      if (!same) {
+#pragma unroll
+        for (int p = 0; p < 4; ++p) xch[4 * r + p] = boff[2 * p + 1];
+      }
+      wave_lds_fence();
+      for (uint32_t t = me; t < slots; t += n_act) xch[t] = load_entry<Tab>(base, xch[t]);
+      wave_lds_fence();
+      if (!same) {
+#pragma unroll
+        for (int p = 0; p < 4; ++p) v[2 * p + 1] = xch[4 * r + p];
+      }
+      wave_lds_fence();  // the exchange is free for the next level
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < 8; ++c) v[c] = load_entry<Tab>(base, boff[c]);
+  }
+}
+
 __device__ __forceinline__ float2 trilinear(const float fr[3], const float2 fv[8]) {
   const float gx = 1.0f - fr[0], gy = 1.0f - fr[1], gz = 1.0f - fr[2];
   float a0 = 0.0f, a1 = 0.0f;
@@ -273,17 +353,15 @@ __device__ __forceinline__ float2 trilinear(const float fr[3], const float2 fv[8
 // a gather is one 32-bit VALU offset plus a scalar-based load.
 template <class Tab, int LA, int LB>
 __device__ __forceinline__ void encode_two(const char *base_a, const char *base_b,
-                                           const float xc[3], float2 &fa, float2 &fb) {
+                                           const float xc[3], float2 &fa, float2 &fb,
+                                           uint32_t *xch) {
   uint32_t oa[8], ob[8];
   float fra[3], frb[3];
   level_corners<Tab, LA>(xc, oa, fra);
   level_corners<Tab, LB>(xc, ob, frb);
   typename RawFeat<Tab>::type va[8], vb[8];
-#pragma unroll
-  for (int c = 0; c < 8; ++c) {
-    va[c] = load_entry<Tab>(base_a, oa[c]);
-    vb[c] = load_entry<Tab>(base_b, ob[c]);
-  }
+  load_corners<Tab, LA>(base_a, oa, va, xch);
+  load_corners<Tab, LB>(base_b, ob, vb, xch);
   float2 fva[8], fvb[8];
 #pragma unroll
   for (int c = 0; c < 8; ++c) {

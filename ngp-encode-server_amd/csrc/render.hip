@@ -289,7 +289,8 @@ template <class Tab, int P>
 __device__ __forceinline__ void encode_part_loop(const typename PosIn<Tab>::type *__restrict__ pos,
                                                  uint32_t n, uint32_t stride, const Tab *table,
                                                  typename PartOut<Tab>::type *__restrict__ feat,
-                                                 uint32_t grp, uint32_t blocks_per_part) {
+                                                 uint32_t grp, uint32_t blocks_per_part,
+                                                 uint32_t *xch) {
   constexpr int LA = P, LB = 15 - P;
   const char *base_a = level_base<Tab, LA>(table), *base_b = level_base<Tab, LB>(table);
   typename PartOut<Tab>::type *dst = feat + (size_t)P * stride;
@@ -297,7 +298,7 @@ __device__ __forceinline__ void encode_part_loop(const typename PosIn<Tab>::type
     float xc[3];
     PosIn<Tab>::get(pos[s], xc);
     float2 fa, fb;
-    encode_two<Tab, LA, LB>(base_a, base_b, xc, fa, fb);
+    encode_two<Tab, LA, LB>(base_a, base_b, xc, fa, fb, xch);
     dst[s] = PartOut<Tab>::pack(fa, fb);
   }
 }
@@ -310,17 +311,19 @@ __global__ __launch_bounds__(256) void encode_parts(const typename PosIn<Tab>::t
                                                     uint32_t stride, const Tab *__restrict__ table,
                                                     typename PartOut<Tab>::type *__restrict__ feat,
                                                     uint32_t blocks_per_part) {
+  __shared__ uint32_t xch_all[4][256];  // per-wavefront gather exchange (FP16 hashed levels)
+  uint32_t *xch = xch_all[threadIdx.x >> 6];
   const uint32_t n = min(*n_ptr, stride);
   const uint32_t grp = blockIdx.x / kParts;
   switch (blockIdx.x % kParts) {
-    case 0: encode_part_loop<Tab, 0>(pos, n, stride, table, feat, grp, blocks_per_part); break;
-    case 1: encode_part_loop<Tab, 1>(pos, n, stride, table, feat, grp, blocks_per_part); break;
-    case 2: encode_part_loop<Tab, 2>(pos, n, stride, table, feat, grp, blocks_per_part); break;
-    case 3: encode_part_loop<Tab, 3>(pos, n, stride, table, feat, grp, blocks_per_part); break;
-    case 4: encode_part_loop<Tab, 4>(pos, n, stride, table, feat, grp, blocks_per_part); break;
-    case 5: encode_part_loop<Tab, 5>(pos, n, stride, table, feat, grp, blocks_per_part); break;
-    case 6: encode_part_loop<Tab, 6>(pos, n, stride, table, feat, grp, blocks_per_part); break;
-    default: encode_part_loop<Tab, 7>(pos, n, stride, table, feat, grp, blocks_per_part); break;
+    case 0: encode_part_loop<Tab, 0>(pos, n, stride, table, feat, grp, blocks_per_part, xch); break;
+    case 1: encode_part_loop<Tab, 1>(pos, n, stride, table, feat, grp, blocks_per_part, xch); break;
+    case 2: encode_part_loop<Tab, 2>(pos, n, stride, table, feat, grp, blocks_per_part, xch); break;
+    case 3: encode_part_loop<Tab, 3>(pos, n, stride, table, feat, grp, blocks_per_part, xch); break;
+    case 4: encode_part_loop<Tab, 4>(pos, n, stride, table, feat, grp, blocks_per_part, xch); break;
+    case 5: encode_part_loop<Tab, 5>(pos, n, stride, table, feat, grp, blocks_per_part, xch); break;
+    case 6: encode_part_loop<Tab, 6>(pos, n, stride, table, feat, grp, blocks_per_part, xch); break;
+    default: encode_part_loop<Tab, 7>(pos, n, stride, table, feat, grp, blocks_per_part, xch); break;
   }
 }
 
