@@ -10,8 +10,11 @@ pytestmark = pytest.mark.gpu
 FP32, MIXED = 0, 1
 # FP32 path: per-pixel colour within 2e-5 and depth within 1e-4 of FP64.
 FP32_RGB_TOL, FP32_DEPTH_TOL = 2e-5, 1e-4
-# Mixed path (FP16 table + FP16 MFMA MLPs): >= 45 dB PSNR on the float image.
+# Mixed path (FP16 table + FP16 MFMA MLPs): >= 45 dB PSNR on the float image
+# and the expected depth within 1e-4 (measured: 1.2e-5 on the stereo 1080p
+# frame of the seeded scene).
 MIXED_PSNR_DB = 45.0
+MIXED_DEPTH_TOL = 1e-4
 
 
 def psnr(a, b):
@@ -45,6 +48,7 @@ def test_full_image_160x120(nes, oracle, gpu_ctx, scene1, prec):
         assert psnr(got, rgb) >= MIXED_PSNR_DB
         hit = ns > 0
         assert psnr(got[hit], rgb[hit]) >= MIXED_PSNR_DB - 6
+        assert np.abs(out["depth"].reshape(-1) - depth).max() < MIXED_DEPTH_TOL
     q = np.array([[oracle.lib().orc_quantize(c) for c in px] for px in rgb[::97]])
     assert np.abs(out["rgb24"].reshape(-1, 3)[::97].astype(int) - q).max() <= 1
 
@@ -58,9 +62,11 @@ def test_640x480_fp32_and_mixed_subset(nes, oracle, gpu_ctx, scene1, prec):
     assert np.array_equal(out["n_samples"].reshape(-1)[pix], ns)
     got = out["rgb"].reshape(-1, 3)[pix]
     p = psnr(got, rgb)
+    derr = np.abs(out["depth"].reshape(-1)[pix] - depth).max()
     print(f"precision {prec}: PSNR {p:.1f} dB vs FP64 oracle on {len(pix)} pixels, "
-          f"max depth err {np.abs(out['depth'].reshape(-1)[pix] - depth).max():.2e}")
+          f"max depth err {derr:.2e}")
     assert p >= (90.0 if prec == FP32 else MIXED_PSNR_DB)
+    assert derr < (FP32_DEPTH_TOL if prec == FP32 else MIXED_DEPTH_TOL)
 
 
 @pytest.mark.parametrize("prec", [FP32, MIXED])
@@ -85,6 +91,48 @@ def test_mixed_resolution_batch(nes, oracle, gpu_ctx):
     for v, o in zip(views, outs):
         (solo,), _ = gpu_ctx.render(v, precision=MIXED)
         assert np.array_equal(solo["rgb"], o["rgb"]) and np.array_equal(solo["depth"], o["depth"])
+
+
+def _contrast_scene(oracle):
+    """The seeded scene with the table scaled by 30 and the density output
+    row by 4: densities from ~1e-3 to ~1e3 and colours across [0, 1] (the
+    seeded scene's colours sit near sigmoid(0)), so colour and depth errors
+    of the FP16 path are not hidden by a nearly constant image."""
+    t, w, o = oracle.scene(1)
+    t = (t * 30.0).astype(np.float32)
+    w = w.copy()
+    w[2048:2048 + 64] *= 4.0
+    return t, w, o
+
+
+@pytest.mark.parametrize("prec", [FP32, MIXED])
+def test_high_contrast_scene(nes, oracle, prec):
+    scene = _contrast_scene(oracle)
+    cam, ocam = cams(nes, oracle, 160, 120)
+    with nes.Context(*scene) as ctx:
+        (out,), _ = ctx.render(cam, precision=prec)
+    pix = np.arange(160 * 120, dtype=np.uint32)
+    rgb, depth, ns = oracle.render_pixels(*scene, ocam, pix, threads=16)
+    hit = ns > 0
+    # the image really is high-contrast
+    assert rgb[hit].std() > 0.1 and np.ptp(depth[hit]) > 0.5
+    got = out["rgb"].reshape(-1, 3)
+    # FP32 visits identical sample sets.  On the FP16 path a ray whose
+    # transmittance crosses 1e-4 within the rounding of its FP16 log-density
+    # stops a few samples earlier or later (measured: 10 of 19,200 pixels, at
+    # most 3 samples); those pixels carry the largest depth errors.
+    dn = np.abs(out["n_samples"].reshape(-1).astype(int) - ns)
+    p = psnr(got, rgb)
+    derr = np.abs(out["depth"].reshape(-1) - depth)
+    print(f"precision {prec}: high-contrast PSNR {p:.1f} dB, depth err max {derr.max():.2e} "
+          f"p99 {np.percentile(derr, 99):.2e}, sample-count changes {(dn > 0).sum()}")
+    if prec == FP32:
+        assert dn.max() == 0
+        assert np.abs(got - rgb).max() < 1e-4 and derr.max() < 1e-3
+    else:
+        assert (dn > 0).mean() < 0.005 and dn.max() <= 4
+        assert p >= MIXED_PSNR_DB
+        assert np.percentile(derr, 99) < 1e-3 and derr.max() < 2e-2
 
 
 def _const_scene(sigma):
@@ -144,9 +192,10 @@ def test_stereo_1080p_subset(nes, oracle, gpu_ctx, scene1, prec):
         rgb, depth, ns = oracle_view(oracle, scene1, oc, pix)
         assert np.array_equal(o["n_samples"].reshape(-1)[pix], ns)
         p = psnr(o["rgb"].reshape(-1, 3)[pix], rgb)
-        print(f"precision {prec}: stereo eye PSNR {p:.1f} dB, max depth err "
-              f"{np.abs(o['depth'].reshape(-1)[pix] - depth).max():.2e}")
+        derr = np.abs(o["depth"].reshape(-1)[pix] - depth).max()
+        print(f"precision {prec}: stereo eye PSNR {p:.1f} dB, max depth err {derr:.2e}")
         assert p >= (90.0 if prec == FP32 else MIXED_PSNR_DB)
+        assert derr < (FP32_DEPTH_TOL if prec == FP32 else MIXED_DEPTH_TOL)
 
 
 def test_context_budget_bounds_render_buffers(nes, scene1):

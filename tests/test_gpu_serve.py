@@ -35,17 +35,21 @@ def test_alone_equals_mix_equals_direct(nes, gpu_ctx):
 
 
 def test_latest_pose_wins(nes, gpu_ctx):
+    """A client's newer pose replaces its waiting one (admission held, so the
+    first is certainly still waiting); other clients' requests are kept."""
     with nes.Server(gpu_ctx, max_inflight=1) as srv:
+        srv.hold(True)
         busy = srv.submit(0, stereo(nes, 2064, 2208, 10.0))
         first = srv.submit(1, stereo(nes, 640, 480, 50.0, index=1))
         second = srv.submit(1, stereo(nes, 640, 480, 60.0, index=2))
+        srv.hold(False)
         rc1, _, _ = srv.wait(first)
         rc2, frames, res = srv.wait(second)
         assert srv.wait(busy)[0] == 0
         st = srv.stats()
     assert rc2 == 0 and res.index == 2 and len(frames) == 2
-    assert rc1 in (0, nes.NES_ECANCELED)
-    assert st.superseded == (1 if rc1 == nes.NES_ECANCELED else 0)
+    assert rc1 == nes.NES_ECANCELED
+    assert st.superseded == 1 and st.completed == 2
 
 
 def test_invalid_requests_rejected(nes, gpu_ctx):
