@@ -163,9 +163,14 @@ def ceilings_leg(N, samples, ms_encode, l2_req_per_launch, l1_acc_per_launch=Non
     encode's rates against them."""
     l1, _ = N.gather_rate(16 << 10)
     l2, _ = N.gather_rate(2 << 20)
-    lane_loads = 128 * samples / (ms_encode * 1e-3)  # 16 levels x 8 corners
+    # corner fetches: 16 levels x 8 corners per sample.  The encode issues
+    # fewer lane addresses than that (one per corner pair on dense levels,
+    # one per 16 B chunk on hashed levels 5-13), so this can exceed the
+    # lane-address ceiling.
+    fetches = 128 * samples / (ms_encode * 1e-3)
     out = {"l1_lane_loads_per_s": l1, "l2_requests_per_s": l2,
-           "encode_lane_loads_per_s": lane_loads, "encode_l1_frac": round(lane_loads / l1, 4)}
+           "encode_corner_fetches_per_s": fetches,
+           "encode_fetches_per_l1_lane_slot": round(fetches / l1, 4)}
     if l1_acc_per_launch:
         out["encode_l1_accesses_frac"] = round(l1_acc_per_launch / (ms_encode * 1e-3) / l1, 4)
     if l2_req_per_launch:
@@ -360,11 +365,12 @@ def main():
             "kernel": "encode_parts<f16>", "bound": "hbm",
             "achieved": round(gathered_gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
             "frac": round(gathered_gbs / PEAK_HBM_GBS, 4), "traffic": traffic,
-            # what the counters show: the gathers are L1/L2 hits; XCD 0 (level
-            # 15, no reuse) runs at the L2 request rate, the others at the
-            # vector-L1 lane-address rate; HBM carries far less than 512 B/sample
-            "limiter": "L2 request rate of XCD 0 (level 15) and vector-L1 lane-address rate "
-                       "(XCDs 1-7); HBM is not the limiter",
+            # what the counters and per-XCD timings show: the gathers are L1/L2
+            # hits, each XCD is bound by the vector L1's gather slots (one lane
+            # address per lane and instruction), XCD 0 (level 15, no reuse)
+            # also by its L2 request rate; HBM carries far less than 512 B/sample
+            "limiter": "vector-L1 gather slots on every XCD, and the L2 request rate of XCD 0 "
+                       "(level 15); HBM is not the limiter",
         }
         if traffic:
             hbm = traffic / (ms_enc * 1e-3) / 1e9
