@@ -485,10 +485,26 @@ __global__ __launch_bounds__(256) void train_dcomposite(
 // they cost 2x and 1.5x any other level.  Parts 0 and 1 therefore sum their
 // dense level in LDS (39 / 97 KB) and add the workgroup's totals once at the
 // end, 16 lanes to a line; the grid is a fixed kScatterBlocks workgroups per
-// part so those flushes stay few.
+// part so those flushes stay few.  Levels 2 (29,791 entries, 238 KB) and 3
+// (79,507, 636 KB) do not fit; parts 2 and 3 run their workgroups in groups
+// of 2 and 4, each summing one slice of the level over all samples and
+// taking the level's partner (13, 12) for every 2nd / 4th block of samples.
+// The LDS array is sized for the largest slice, so every workgroup of the
+// kernel holds one CU's LDS (measured on the 2^18-ray step: levels 0-1 in
+// LDS 4.52 ms, only level 0 4.70, none 5.05, levels 0-2 4.22).
 constexpr uint32_t kScatterBlocks = 256;
-constexpr uint32_t kLevel0Floats = 2 * (kLevelOffset[1] - kLevelOffset[0]);
-constexpr uint32_t kLevel1Floats = 2 * (kLevelOffset[2] - kLevelOffset[1]);
+__host__ __device__ constexpr uint32_t dense_floats(int l) {
+  return 2 * (kLevelOffset[l + 1] - kLevelOffset[l]);
+}
+__host__ __device__ constexpr uint32_t dense_slices(int l) { return l == 2 ? 2u : l == 3 ? 4u : 1u; }
+__host__ __device__ constexpr uint32_t dense_slice_floats(int l) {
+  return (dense_floats(l) + dense_slices(l) - 1) / dense_slices(l);
+}
+constexpr uint32_t kDenseLdsFloats = dense_slice_floats(3) > dense_slice_floats(2)
+    ? (dense_slice_floats(3) > dense_slice_floats(1) ? dense_slice_floats(3) : dense_slice_floats(1))
+    : (dense_slice_floats(2) > dense_slice_floats(1) ? dense_slice_floats(2) : dense_slice_floats(1));
+static_assert(kDenseLdsFloats * 4 <= 160 * 1024, "dense-level slice exceeds the LDS");
+static_assert(kScatterBlocks % 4 == 0, "workgroup groups of 4");
 
 __global__ __launch_bounds__(256) void scatter_parts(const float4 *__restrict__ pos,
                                                      const float4 *__restrict__ dfeat,
@@ -496,25 +512,31 @@ __global__ __launch_bounds__(256) void scatter_parts(const float4 *__restrict__ 
                                                      uint32_t stride, float *grad_table,
                                                      uint32_t blocks_per_part) {
   constexpr uint32_t kPerBlock = 256 / 4;
-  __shared__ float dense_sum[kLevel1Floats];
+  __shared__ float dense_sum[kDenseLdsFloats];
   const uint32_t n = min(*n_ptr, stride);
   const int part = (int)(blockIdx.x % kParts);
   const uint32_t grp = blockIdx.x / kParts;
   const PartLevels lp = part_levels(part);
   const uint32_t xb = (threadIdx.x >> 1) & 1u, ft = threadIdx.x & 1u;
-  const bool in_lds = part <= 1;  // block-uniform: level 0 (part 0) or 1 (part 1) in LDS
-  const uint32_t nfl = part == 0 ? kLevel0Floats : kLevel1Floats;
-  const uint32_t off = part == 0 ? 0u : 2u * kDevLevelOffset[1];
+  // block-uniform: the dense level of parts 0-3 is summed in LDS, in slices
+  const bool in_lds = part <= 3;
+  const uint32_t ns = in_lds ? dense_slices(part) : 1u;
+  const uint32_t sub = grp % ns, g_idx = grp / ns, g_cnt = blocks_per_part / ns;
+  const uint32_t slice = in_lds ? dense_slice_floats(part) : 0u;
+  const uint32_t nfl = in_lds ? min(slice, dense_floats(part) - sub * slice) : 0u;
+  const uint32_t off = in_lds ? 2u * kDevLevelOffset[part] + sub * slice : 0u;
   if (in_lds)
     for (uint32_t i = threadIdx.x; i < nfl; i += blockDim.x) dense_sum[i] = 0.0f;
   __syncthreads();
-  for (uint32_t s = grp * kPerBlock + (threadIdx.x >> 2); s < n;
-       s += blocks_per_part * kPerBlock) {
+  for (uint32_t s = g_idx * kPerBlock + (threadIdx.x >> 2); s < n; s += g_cnt * kPerBlock) {
     const float4 p = pos[s];
     const float4 df = dfeat[(size_t)part * stride + s];
     const float xc[3] = {p.x, p.y, p.z};
+    // sliced parts: the fine level of a block of samples goes to one of the group
+    const bool fine_here = ns == 1 || (s / kPerBlock) % ns == sub;
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
+      if (q == 1 && !fine_here) continue;
       const float d0 = q == 0 ? df.x : df.z, d1 = q == 0 ? df.y : df.w;
       if (d0 == 0.0f && d1 == 0.0f) continue;
       const float dv = ft ? d1 : d0;
@@ -536,10 +558,12 @@ __global__ __launch_bounds__(256) void scatter_parts(const float4 *__restrict__ 
         const uint32_t hash = (x ^ (y * kPrime1) ^ (z * kPrime2)) & ((1u << kLog2T) - 1u);
         const uint32_t e = lp.off[q] + (lp.dense[q] ? dense : hash);
         const float w = wx * ((c & 1) ? fr[1] : 1.0f - fr[1]) * ((c >> 1) ? fr[2] : 1.0f - fr[2]);
-        if (in_lds && q == 0)
-          atomicAdd(&dense_sum[2 * e + ft - off], w * dv);
-        else
+        if (in_lds && q == 0) {
+          const uint32_t li = 2 * e + ft - off;  // wraps above nfl outside this half
+          if (li < nfl) atomicAdd(&dense_sum[li], w * dv);
+        } else {
           unsafeAtomicAdd(grad_table + 2 * (size_t)e + ft, w * dv);
+        }
       }
     }
   }

@@ -127,11 +127,14 @@ def test_batch_loss_matches_oracle_and_adam_first_step(nes, oracle, scene1):
         assert abs(loss - loss_ref) <= 1e-4 * loss_ref
         # whole batch, kinks included: relative L2 per layer.  The student's
         # layer-1 gradient is a sum of ~1e5 terms of random sign, so one
-        # flipped kink moves it by ~1/sqrt(1e5) = 3e-3 of its norm.
+        # ReLU pre-activation on the other side of zero in FP32 than in FP64
+        # would move it by ~1/sqrt(1e5) = 3e-3 of its norm; the forward is
+        # deterministic, and on this batch none does (measured: <= 3e-6 per
+        # layer, the atomics' summation order), so 1e-3 still catches one.
         for a, b in [(0, 2048), (2048, 3072), (3072, 5056), (5056, 9152), (9152, 9344)]:
             rel = np.linalg.norm(gw[a:b] - gw_ref[a:b]) / np.linalg.norm(gw_ref[a:b])
             print(f"batch grad {a}:{b} rel-l2 {rel:.2e}", flush=True)
-            assert rel <= 2e-2, (a, b, rel)
+            assert rel <= 1e-3, (a, b, rel)
         l0, st = tr.step()
         assert abs(l0 - loss) <= 1e-5 * loss and st.step == 1
         t1, w1 = tr.get_params()
