@@ -544,6 +544,27 @@ static double percentile(std::vector<double> l, double p) {
 
 }  // namespace nes
 
+namespace nes {
+
+// The slots' device buffers, events and streams and the render stream.
+static void server_free_device(nes_server *s) {
+  if (s->render_stream) hipStreamSynchronize(s->render_stream);
+  for (auto &sl : s->slots) {
+    if (sl->stream) hipStreamSynchronize(sl->stream);
+    render_work_free(sl->work);
+    sl->yuv.release();
+    sl->ovl_place.release();
+    if (sl->ev_done) hipEventDestroy(sl->ev_done);
+    if (sl->ev_rendered) hipEventDestroy(sl->ev_rendered);
+    if (sl->stream) hipStreamDestroy(sl->stream);
+  }
+  s->slots.clear();
+  if (s->render_stream) hipStreamDestroy(s->render_stream);
+  s->render_stream = nullptr;
+}
+
+}  // namespace nes
+
 extern "C" {
 
 int nes_server_create(nes_ctx *ctx, const nes_server_config *cfg, nes_server **out) {
@@ -577,23 +598,30 @@ int nes_server_create(nes_ctx *ctx, const nes_server_config *cfg, nes_server **o
   const uint64_t share = s->cfg.hbm_budget_bytes / n_slots;
   // the YUV staging and overlay buffers of a slot sit outside its render work
   const uint64_t staging = (uint64_t)s->cfg.max_batch_rays * (depth ? 4 : 2) + (2u << 20);
-  NES_HIP_TRY(hipStreamCreateWithFlags(&s->render_stream, hipStreamNonBlocking));
-  for (uint32_t i = 0; i < n_slots; ++i) {
-    auto sl = std::make_unique<Slot>();
-    sl->work.budget = share - staging;
-    // Allocate the slot's buffers for the largest batch now: growing one
-    // later frees the old buffer, and hipFree waits for the whole device,
-    // draining the render stream while the scheduler thread is blocked.
-    if (per_slot <= sl->work.budget) {
-      NES_TRY(render_work_reserve(sl->work, s->cfg.max_batch_rays, s->cfg.max_batch_rays,
-                                  kPinnedViews, s->cfg.precision));
-      // YUV420P is 1.5 bytes per pixel, twice with the depth stream
-      NES_TRY(sl->yuv.reserve((size_t)s->cfg.max_batch_rays * 3 / 2 * (depth ? 2 : 1) + 256));
+  auto build = [&]() -> int {
+    NES_HIP_TRY(hipStreamCreateWithFlags(&s->render_stream, hipStreamNonBlocking));
+    for (uint32_t i = 0; i < n_slots; ++i) {
+      s->slots.push_back(std::make_unique<Slot>());
+      Slot &sl = *s->slots.back();
+      sl.work.budget = share - staging;
+      NES_HIP_TRY(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking));
+      NES_HIP_TRY(hipEventCreateWithFlags(&sl.ev_done, hipEventDisableTiming));
+      NES_HIP_TRY(hipEventCreateWithFlags(&sl.ev_rendered, hipEventDisableTiming));
+      // Allocate the slot's buffers for the largest batch now: growing one
+      // later frees the old buffer, and hipFree waits for the whole device,
+      // draining the render stream while the scheduler thread is blocked.
+      if (per_slot <= sl.work.budget) {
+        NES_TRY(render_work_reserve(sl.work, s->cfg.max_batch_rays, s->cfg.max_batch_rays,
+                                    kPinnedViews, s->cfg.precision));
+        // YUV420P is 1.5 bytes per pixel, twice with the depth stream
+        NES_TRY(sl.yuv.reserve((size_t)s->cfg.max_batch_rays * 3 / 2 * (depth ? 2 : 1) + 256));
+      }
     }
-    NES_HIP_TRY(hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking));
-    NES_HIP_TRY(hipEventCreateWithFlags(&sl->ev_done, hipEventDisableTiming));
-    NES_HIP_TRY(hipEventCreateWithFlags(&sl->ev_rendered, hipEventDisableTiming));
-    s->slots.push_back(std::move(sl));
+    return NES_OK;
+  };
+  if (const int rc = build(); rc != NES_OK) {
+    server_free_device(s.get());  // what was created before the failure
+    return rc;
   }
   nes_server *raw = s.release();
   raw->scheduler = std::thread(scheduler_loop, raw);
@@ -615,17 +643,7 @@ int nes_server_destroy(nes_server *s) {
   }
   s->cv_work.notify_all();
   if (s->scheduler.joinable()) s->scheduler.join();
-  if (s->render_stream) hipStreamSynchronize(s->render_stream);
-  for (auto &sl : s->slots) {
-    hipStreamSynchronize(sl->stream);
-    render_work_free(sl->work);
-    sl->yuv.release();
-    sl->ovl_place.release();
-    hipEventDestroy(sl->ev_done);
-    hipEventDestroy(sl->ev_rendered);
-    hipStreamDestroy(sl->stream);
-  }
-  if (s->render_stream) hipStreamDestroy(s->render_stream);
+  server_free_device(s);
   for (auto &kv : s->tickets) s->pinned.put(kv.second->yuv);
   delete s;
   return NES_OK;

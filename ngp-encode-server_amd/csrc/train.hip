@@ -749,52 +749,59 @@ int nes_trainer_create(const nes_scene_desc *init, const nes_train_config *cfg,
   if (!t->cfg.fov_y_deg) t->cfg.fov_y_deg = 60.0f;
   if (t->cfg.rays_per_batch > kMaxSlots) return NES_ERANGE;
   const bool have_targets = pose_c2w && targets && t->cfg.n_poses && t->cfg.width && t->cfg.height;
-  t->n_params = t->n_table + kWeights;
-  NES_HIP_TRY(hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking));
-  NES_HIP_TRY(hipEventCreate(&t->ev0));
-  NES_HIP_TRY(hipEventCreate(&t->ev1));
-  for (float **p : {&t->params, &t->grads, &t->m, &t->v})
-    NES_HIP_TRY(hipMalloc(p, t->n_params * sizeof(float)));
-  NES_HIP_TRY(hipMalloc(&t->occ, NES_OCC_WORDS * sizeof(uint32_t)));
-  NES_HIP_TRY(hipMalloc(&t->loss, 64));
-  const size_t nr = t->cfg.rays_per_batch;
-  NES_HIP_TRY(hipMalloc(&t->target, nr * 3 * sizeof(float)));
-  NES_TRY(t->dCb.reserve(nr * 3 * sizeof(float)));
-  t->dC = t->dCb.as<float>();
-  // the padding between levels stays zero (gradients and moments there stay 0)
-  NES_HIP_TRY(hipMemset(t->params, 0, t->n_params * sizeof(float)));
-  NES_TRY(table_copy(t->params, true, init->table, false, sizeof(float), hipMemcpyHostToDevice, nullptr));
-  NES_HIP_TRY(hipMemcpy(t->params + t->n_table, init->weights, kWeights * sizeof(float),
-                        hipMemcpyHostToDevice));
-  NES_HIP_TRY(hipMemcpy(t->occ, init->occupancy, NES_OCC_WORDS * sizeof(uint32_t),
-                        hipMemcpyHostToDevice));
-  for (float *p : {t->grads, t->m, t->v})
-    zero_f32<<<1024, 256, 0, t->stream>>>(p, t->n_params);
-  if (have_targets) {
-    std::vector<ViewDesc> pv(t->cfg.n_poses);
-    for (uint32_t i = 0; i < t->cfg.n_poses; ++i) {
-      ViewDesc &v = pv[i];
-      std::memset(&v, 0, sizeof(v));
-      std::memcpy(v.c2w, pose_c2w + 12 * i, sizeof(v.c2w));
-      v.focal = (float)(0.5 * (double)t->cfg.height /
-                        std::tan(0.5 * (double)t->cfg.fov_y_deg * 3.14159265358979323846 / 180.0));
-      v.width = t->cfg.width;
-      v.height = t->cfg.height;
+  auto build = [&]() -> int {
+    t->n_params = t->n_table + kWeights;
+    NES_HIP_TRY(hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking));
+    NES_HIP_TRY(hipEventCreate(&t->ev0));
+    NES_HIP_TRY(hipEventCreate(&t->ev1));
+    for (float **p : {&t->params, &t->grads, &t->m, &t->v})
+      NES_HIP_TRY(hipMalloc(p, t->n_params * sizeof(float)));
+    NES_HIP_TRY(hipMalloc(&t->occ, NES_OCC_WORDS * sizeof(uint32_t)));
+    NES_HIP_TRY(hipMalloc(&t->loss, 64));
+    const size_t nr = t->cfg.rays_per_batch;
+    NES_HIP_TRY(hipMalloc(&t->target, nr * 3 * sizeof(float)));
+    NES_TRY(t->dCb.reserve(nr * 3 * sizeof(float)));
+    t->dC = t->dCb.as<float>();
+    // the padding between levels stays zero (gradients and moments there stay 0)
+    NES_HIP_TRY(hipMemset(t->params, 0, t->n_params * sizeof(float)));
+    NES_TRY(table_copy(t->params, true, init->table, false, sizeof(float), hipMemcpyHostToDevice, nullptr));
+    NES_HIP_TRY(hipMemcpy(t->params + t->n_table, init->weights, kWeights * sizeof(float),
+                          hipMemcpyHostToDevice));
+    NES_HIP_TRY(hipMemcpy(t->occ, init->occupancy, NES_OCC_WORDS * sizeof(uint32_t),
+                          hipMemcpyHostToDevice));
+    for (float *p : {t->grads, t->m, t->v})
+      zero_f32<<<1024, 256, 0, t->stream>>>(p, t->n_params);
+    if (have_targets) {
+      std::vector<ViewDesc> pv(t->cfg.n_poses);
+      for (uint32_t i = 0; i < t->cfg.n_poses; ++i) {
+        ViewDesc &v = pv[i];
+        std::memset(&v, 0, sizeof(v));
+        std::memcpy(v.c2w, pose_c2w + 12 * i, sizeof(v.c2w));
+        v.focal = (float)(0.5 * (double)t->cfg.height /
+                          std::tan(0.5 * (double)t->cfg.fov_y_deg * 3.14159265358979323846 / 180.0));
+        v.width = t->cfg.width;
+        v.height = t->cfg.height;
+      }
+      const size_t tb = (size_t)t->cfg.n_poses * t->cfg.width * t->cfg.height * 3 * sizeof(float);
+      NES_HIP_TRY(hipMalloc(&t->poses, pv.size() * sizeof(ViewDesc)));
+      NES_HIP_TRY(hipMalloc(&t->targets, tb));
+      NES_HIP_TRY(hipMemcpy(t->poses, pv.data(), pv.size() * sizeof(ViewDesc), hipMemcpyHostToDevice));
+      NES_HIP_TRY(hipMemcpy(t->targets, targets, tb, hipMemcpyHostToDevice));
     }
-    const size_t tb = (size_t)t->cfg.n_poses * t->cfg.width * t->cfg.height * 3 * sizeof(float);
-    NES_HIP_TRY(hipMalloc(&t->poses, pv.size() * sizeof(ViewDesc)));
-    NES_HIP_TRY(hipMalloc(&t->targets, tb));
-    NES_HIP_TRY(hipMemcpy(t->poses, pv.data(), pv.size() * sizeof(ViewDesc), hipMemcpyHostToDevice));
-    NES_HIP_TRY(hipMemcpy(t->targets, targets, tb, hipMemcpyHostToDevice));
+    NES_HIP_TRY(hipStreamSynchronize(t->stream));
+    return NES_OK;
+  };
+  if (const int rc = build(); rc != NES_OK) {
+    nes_trainer_destroy(t.release());  // what was allocated before the failure
+    return rc;
   }
-  NES_HIP_TRY(hipStreamSynchronize(t->stream));
   *out = t.release();
   return NES_OK;
 }
 
 int nes_trainer_destroy(nes_trainer *t) {
   if (!t) return NES_EINVAL;
-  hipStreamSynchronize(t->stream);
+  if (t->stream) hipStreamSynchronize(t->stream);
   t->dCb.release();
   for (float *p : {t->params, t->grads, t->m, t->v, t->targets, t->target, t->loss})
     if (p) hipFree(p);
@@ -805,9 +812,9 @@ int nes_trainer_destroy(nes_trainer *t) {
   t->xo.release();
   t->xd.release();
   render_work_free(t->work);
-  hipEventDestroy(t->ev0);
-  hipEventDestroy(t->ev1);
-  hipStreamDestroy(t->stream);
+  if (t->ev0) hipEventDestroy(t->ev0);
+  if (t->ev1) hipEventDestroy(t->ev1);
+  if (t->stream) hipStreamDestroy(t->stream);
   delete t;
   return NES_OK;
 }
