@@ -18,13 +18,15 @@
 //                    over samples (tile transposed through LDS), summed over
 //                    the wave's tiles in MFMA accumulators, then reduced in
 //                    LDS and flushed once per workgroup
-//   scatter_parts    XCD-partitioned FP32 atomic adds of dL/dfeature into the
-//                    table gradient (trilinear weights, 8 corners, 2 features)
+//   scatter_slices   dL/dfeature into the table gradient (trilinear weights,
+//                    8 corners, 2 features), one level slice per workgroup
+//                    accumulated in LDS
 //   adam             fused update, zeroes the gradient for the next step
 #include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <vector>
 
 #include "device_common.h"
 
@@ -383,8 +385,27 @@ __global__ void train_rays(TrainRayArgs a) {
 }
 
 // Loss = mean over rays and channels of (C - target)^2; dC = dL/dC.
-__global__ void train_loss(const float *rgb, const float *target, uint32_t n, float *dC,
-                           float *loss) {
+// Sum of the values of one workgroup's lanes (blockDim.x == kLossThreads), in
+// a fixed order; the total is valid in thread 0.
+constexpr int kLossThreads = 256;
+__device__ __forceinline__ float block_sum(float l) {
+  __shared__ float wsum[kLossThreads / 64];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) l += __shfl_xor(l, o);
+  if (lane_id() == 0) wsum[threadIdx.x >> 6] = l;
+  __syncthreads();
+  float t = 0.0f;
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int w = 0; w < kLossThreads / 64; ++w) t += wsum[w];
+  return t;
+}
+
+// dL/dC per ray and one partial loss per workgroup; loss_sum adds the partials
+// in order, so the loss is the same on every run (one float atomic per
+// wavefront into a single address took 55 us for 2^18 rays, these two 10 us).
+__global__ __launch_bounds__(kLossThreads) void train_loss(const float *rgb, const float *target,
+                                                           uint32_t n, float *dC, float *partial) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   float l = 0.0f;
   if (i < n) {
@@ -396,9 +417,16 @@ __global__ void train_loss(const float *rgb, const float *target, uint32_t n, fl
       dC[3 * i + c] = 2.0f * e * norm;
     }
   }
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) l += __shfl_xor(l, o);
-  if (lane_id() == 0 && l != 0.0f) atomicAdd(loss, l);
+  l = block_sum(l);
+  if (threadIdx.x == 0) partial[blockIdx.x] = l;
+}
+
+__global__ __launch_bounds__(kLossThreads) void loss_sum(const float *partial, uint32_t n_part,
+                                                         float *loss) {
+  float l = 0.0f;
+  for (uint32_t i = threadIdx.x; i < n_part; i += kLossThreads) l += partial[i];
+  l = block_sum(l);
+  if (threadIdx.x == 0) *loss = l;
 }
 
 // Per ray: walk the composited samples again and form the gradients with
@@ -469,110 +497,277 @@ __global__ __launch_bounds__(256) void train_dcomposite(
   (void)rays;
 }
 
-// dL/dfeature -> table gradient, XCD-partitioned like encode_parts.
+// dL/dfeature -> table gradient, one slice of one level per workgroup.
 //
-// The device's float-atomic rate is set by memory requests (about 21 G/s for
-// the whole chip), and the adds of one instruction that fall in one 64 B line
-// leave the CU as one request.  So four lanes share a sample: lane bit 1 picks
-// the x corner, bit 0 the feature.  The two x corners of a (y, z) pair are
-// entries e and e + 1 on dense levels and differ only in bits 0-2 of the hash
-// on hashed levels when x % 8 != 7, so the four adds of one instruction's quad
-// land in one line 7 times in 8: about 4.5 requests per sample and level
-// instead of 16.
+// A float atomic to global memory costs one memory request, and the chip
+// retires about 21 G of them per second however they are spread: added with
+// global atomics (4 lanes per sample so that the adds of a (y, z) corner pair
+// leave as one request), the 16 levels of a 2^18-ray step took 2.47 ms, 92 %
+// of that rate.  Instead, every workgroup owns kSliceEntries consecutive
+// entries of one level, sums their gradient in LDS and adds the slice into
+// the table gradient once at the end.  The LDS sums are FP64: an FP32 atomic
+// add in LDS costs about 3 cycles per lane on this GPU, an FP64 one 0.34
+// (measured with every lane of 256 workgroups adding at random addresses),
+// and the wider sums lose nothing to the order of the adds.
 //
-// Levels 0 and 1 (4,913 and 12,167 dense entries) are hit by every sample on
-// a few hundred lines, and adds to one line queue behind each other: alone
-// they cost 2x and 1.5x any other level.  Parts 0 and 1 therefore sum their
-// dense level in LDS (39 / 97 KB) and add the workgroup's totals once at the
-// end, 16 lanes to a line; the grid is a fixed kScatterBlocks workgroups per
-// part so those flushes stay few.  Levels 2 (29,791 entries, 238 KB) and 3
-// (79,507, 636 KB) do not fit; parts 2 and 3 run their workgroups in groups
-// of 2 and 4, each summing one slice of the level over all samples and
-// taking the level's partner (13, 12) for every 2nd / 4th block of samples.
-// The LDS array is sized for the largest slice, so every workgroup of the
-// kernel holds one CU's LDS (measured on the 2^18-ray step: levels 0-1 in
-// LDS 4.52 ms, only level 0 4.70, none 5.05, levels 0-2 4.22).
-constexpr uint32_t kScatterBlocks = 256;
-__host__ __device__ constexpr uint32_t dense_floats(int l) {
-  return 2 * (kLevelOffset[l + 1] - kLevelOffset[l]);
-}
-__host__ __device__ constexpr uint32_t dense_slices(int l) { return l == 2 ? 2u : l == 3 ? 4u : 1u; }
-__host__ __device__ constexpr uint32_t dense_slice_floats(int l) {
-  return (dense_floats(l) + dense_slices(l) - 1) / dense_slices(l);
-}
-constexpr uint32_t kDenseLdsFloats = dense_slice_floats(3) > dense_slice_floats(2)
-    ? (dense_slice_floats(3) > dense_slice_floats(1) ? dense_slice_floats(3) : dense_slice_floats(1))
-    : (dense_slice_floats(2) > dense_slice_floats(1) ? dense_slice_floats(2) : dense_slice_floats(1));
-static_assert(kDenseLdsFloats * 4 <= 160 * 1024, "dense-level slice exceeds the LDS");
-static_assert(kScatterBlocks % 4 == 0, "workgroup groups of 4");
+// slice_keys first stores, per sample and level, the slice of each of the
+// four (y, z) corner pairs, one byte each (hashed levels: bits 13-18 of
+// y P1 ^ z P2, which x < 2^13 leaves alone; dense levels: the slice of the
+// pair's x corner, with kKeyStraddle when its x + 1 corner opens the next
+// slice).  A slice's workgroup then reads 4 bytes per sample, tests them
+// against its slice with a SWAR byte match, queues the matching samples per
+// wavefront in LDS and processes them 64 at a time: only those load their
+// position and feature gradient.
+//
+// The hashed levels (64 slices each) walk all samples and own their entries
+// outright, so their final add is a plain load and store; the dense levels,
+// whose few slices every sample hits, split the samples over kDenseGroups
+// workgroups per slice, each adding its partial slice with global atomics.
+// Workgroup b runs on XCD b % 8; a hashed slice runs on the XCD named by its
+// level's part (levels p and 15 - p on XCD p), whose L2 holds that feature
+// plane, and the lists are then evened out from their tails.
+constexpr uint32_t kSliceEntries = 8192;  // 128 KB of FP64 feature pairs
+constexpr int kSliceShift = 13;
+constexpr int kSliceThreads = 1024;
+constexpr int kSliceWaves = kSliceThreads / 64;
+constexpr uint32_t kSliceQueue = 256;  // queued samples per wavefront (a ring)
+constexpr uint32_t kSliceTasksPerXcd = 128;
+constexpr uint32_t kKeyStraddle = 0x40u;
+constexpr uint32_t kKeyNone = 0xffffffffu;  // matches no slice
+constexpr uint32_t kDenseGroups[kDenseLevels] = {32, 16, 8, 4, 2};
+static_assert(kSliceEntries == 1u << kSliceShift, "slice size");
+static_assert(((kLevelOffset[kDenseLevels] - kLevelOffset[kDenseLevels - 1]) >> kSliceShift) <
+                  kKeyStraddle - 1u,
+              "dense slice ids fit below the straddle flag, and below kKeyNone's masked byte");
+static_assert(kLevelRes[kLevels - 1] < (int)kSliceEntries,
+              "x and x + 1 differ below the slice bits of a hash");
+static_assert(kSliceQueue >= 128 && (kSliceQueue & (kSliceQueue - 1)) == 0,
+              "a ring of at least 63 waiting + 64 new samples");
 
-__global__ __launch_bounds__(256) void scatter_parts(const float4 *__restrict__ pos,
-                                                     const float4 *__restrict__ dfeat,
-                                                     const uint32_t *__restrict__ n_ptr,
-                                                     uint32_t stride, float *grad_table,
-                                                     uint32_t blocks_per_part) {
-  constexpr uint32_t kPerBlock = 256 / 4;
-  __shared__ float dense_sum[kDenseLdsFloats];
+// Keys of one level: a 16-byte aligned row per level.
+__host__ __device__ constexpr uint32_t key_stride(uint32_t stride) { return (stride + 3u) & ~3u; }
+
+struct SliceTask {
+  uint32_t level;           // kLevels or more marks an empty slot
+  uint32_t slice;           // entries [slice * kSliceEntries, + count) of the level
+  uint32_t count;
+  uint32_t split, n_split;  // samples [n * split / n_split, n * (split + 1) / n_split)
+};
+
+std::vector<SliceTask> slice_tasks() {
+  std::vector<std::vector<SliceTask>> xcd(kParts);
+  for (int l = 0; l < kLevels; ++l) {
+    const uint32_t size = kLevelOffset[l + 1] - kLevelOffset[l];
+    const uint32_t groups = l < kDenseLevels ? kDenseGroups[l] : 1u;
+    const int home = min(l, kLevels - 1 - l);
+    for (uint32_t k = 0; k * kSliceEntries < size; ++k)
+      for (uint32_t g = 0; g < groups; ++g)
+        xcd[home].push_back(SliceTask{(uint32_t)l, k, std::min(kSliceEntries, size - k * kSliceEntries),
+                                      g, groups});
+  }
+  // hashed slices first on every XCD (they run longest), then even out the lists
+  for (auto &v : xcd)
+    std::stable_sort(v.begin(), v.end(), [](const SliceTask &x, const SliceTask &y) {
+      return (x.level >= (uint32_t)kDenseLevels) > (y.level >= (uint32_t)kDenseLevels);
+    });
+  for (;;) {
+    auto lo = std::min_element(xcd.begin(), xcd.end(), [](auto &x, auto &y) { return x.size() < y.size(); });
+    auto hi = std::max_element(xcd.begin(), xcd.end(), [](auto &x, auto &y) { return x.size() < y.size(); });
+    if (hi->size() <= lo->size() + 1) break;
+    lo->push_back(hi->back());
+    hi->pop_back();
+  }
+  std::vector<SliceTask> t(kParts * kSliceTasksPerXcd, SliceTask{2 * kLevels, 0, 0, 0, 0});
+  for (int p = 0; p < kParts; ++p) {
+    if (xcd[p].size() > kSliceTasksPerXcd) return {};
+    std::copy(xcd[p].begin(), xcd[p].end(), t.begin() + p * kSliceTasksPerXcd);
+  }
+  return t;
+}
+
+// Per sample, per level: the slice of each (y, z) corner pair, one byte each.
+__global__ void slice_keys(const float4 *__restrict__ pos, const uint32_t *__restrict__ n_ptr,
+                           uint32_t stride, uint32_t *__restrict__ keys) {
   const uint32_t n = min(*n_ptr, stride);
-  const int part = (int)(blockIdx.x % kParts);
-  const uint32_t grp = blockIdx.x / kParts;
-  const PartLevels lp = part_levels(part);
-  const uint32_t xb = (threadIdx.x >> 1) & 1u, ft = threadIdx.x & 1u;
-  // block-uniform: the dense level of parts 0-3 is summed in LDS, in slices
-  const bool in_lds = part <= 3;
-  const uint32_t ns = in_lds ? dense_slices(part) : 1u;
-  const uint32_t sub = grp % ns, g_idx = grp / ns, g_cnt = blocks_per_part / ns;
-  const uint32_t slice = in_lds ? dense_slice_floats(part) : 0u;
-  const uint32_t nfl = in_lds ? min(slice, dense_floats(part) - sub * slice) : 0u;
-  const uint32_t off = in_lds ? 2u * kDevLevelOffset[part] + sub * slice : 0u;
-  if (in_lds)
-    for (uint32_t i = threadIdx.x; i < nfl; i += blockDim.x) dense_sum[i] = 0.0f;
-  __syncthreads();
-  for (uint32_t s = g_idx * kPerBlock + (threadIdx.x >> 2); s < n; s += g_cnt * kPerBlock) {
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
     const float4 p = pos[s];
-    const float4 df = dfeat[(size_t)part * stride + s];
-    const float xc[3] = {p.x, p.y, p.z};
-    // sliced parts: the fine level of a block of samples goes to one of the group
-    const bool fine_here = ns == 1 || (s / kPerBlock) % ns == sub;
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      if (q == 1 && !fine_here) continue;
-      const float d0 = q == 0 ? df.x : df.z, d1 = q == 0 ? df.y : df.w;
-      if (d0 == 0.0f && d1 == 0.0f) continue;
-      const float dv = ft ? d1 : d0;
-      uint32_t i0[3];
-      float fr[3];
+    for (int l = 0; l < kLevels; ++l) {
+      const float scale = (float)kLevelRes[l];
+      const uint32_t y = (uint32_t)fminf(floorf(p.y * scale), scale - 1.0f);
+      const uint32_t z = (uint32_t)fminf(floorf(p.z * scale), scale - 1.0f);
+      uint32_t key = 0;
+      if (l < kDenseLevels) {
+        const uint32_t n1 = (uint32_t)kLevelRes[l] + 1u;
+        const uint32_t x = (uint32_t)fminf(floorf(p.x * scale), scale - 1.0f);
 #pragma unroll
-      for (int ax = 0; ax < 3; ++ax) {
-        const float pp = xc[ax] * lp.scale[q];
-        const float fl = fminf(floorf(pp), lp.scale[q] - 1.0f);
-        i0[ax] = (uint32_t)fl;
-        fr[ax] = pp - fl;
-      }
-      const uint32_t x = i0[0] + xb;
-      const float wx = xb ? fr[0] : 1.0f - fr[0];
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        const uint32_t y = i0[1] + (c & 1), z = i0[2] + (c >> 1);
-        const uint32_t dense = x + y * lp.n1[q] + z * lp.n1sq[q];
-        const uint32_t hash = (x ^ (y * kPrime1) ^ (z * kPrime2)) & ((1u << kLog2T) - 1u);
-        const uint32_t e = lp.off[q] + (lp.dense[q] ? dense : hash);
-        const float w = wx * ((c & 1) ? fr[1] : 1.0f - fr[1]) * ((c >> 1) ? fr[2] : 1.0f - fr[2]);
-        if (in_lds && q == 0) {
-          const uint32_t li = 2 * e + ft - off;  // wraps above nfl outside this half
-          if (li < nfl) atomicAdd(&dense_sum[li], w * dv);
-        } else {
-          unsafeAtomicAdd(grad_table + 2 * (size_t)e + ft, w * dv);
+        for (int c = 0; c < 4; ++c) {
+          const uint32_t e = x + (y + (c & 1)) * n1 + (z + (c >> 1)) * n1 * n1;
+          const uint32_t b = (e >> kSliceShift) |
+                             ((e & (kSliceEntries - 1u)) == kSliceEntries - 1u ? kKeyStraddle : 0u);
+          key |= b << (8 * c);
         }
+      } else {
+        const uint32_t hy0 = y * kPrime1, hz0 = z * kPrime2;
+        const uint32_t hy[2] = {hy0, hy0 + kPrime1}, hz[2] = {hz0, hz0 + kPrime2};
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+          key |= (((hy[c & 1] ^ hz[c >> 1]) >> kSliceShift) & ((1u << (kLog2T - kSliceShift)) - 1u))
+                 << (8 * c);
+      }
+      keys[(size_t)l * key_stride(stride) + s] = key;
+    }
+  }
+}
+
+// One queued sample of level L into the LDS slice (entries [first, first +
+// count) of the level): the corners of its pairs that fall in the slice.
+template <int L>
+__device__ __forceinline__ void slice_sample(const float4 *__restrict__ pos,
+                                             const float4 *__restrict__ df_part, uint32_t s,
+                                             uint32_t first, uint32_t count, double *acc) {
+  constexpr float kScale = (float)kLevelRes[L];
+  constexpr bool kDense = L < kDenseLevels;
+  constexpr bool kHi = L >= kParts;  // second level of its part: (z, w) of the plane
+  constexpr uint32_t kN1 = (uint32_t)kLevelRes[L] + 1u;
+  const float4 df = df_part[s];
+  const float d0 = kHi ? df.z : df.x, d1 = kHi ? df.w : df.y;
+  if (d0 == 0.0f && d1 == 0.0f) return;
+  const float4 p = pos[s];
+  const float xc[3] = {p.x, p.y, p.z};
+  uint32_t i0[3];
+  float fr[3];
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) {
+    const float pp = xc[ax] * kScale;
+    const float fl = fminf(floorf(pp), kScale - 1.0f);
+    i0[ax] = (uint32_t)fl;
+    fr[ax] = pp - fl;
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const uint32_t y = i0[1] + (c & 1), z = i0[2] + (c >> 1);
+    uint32_t e0, e1;
+    if (kDense) {
+      e0 = i0[0] + y * kN1 + z * (kN1 * kN1) - first;  // wraps above count outside the slice
+      e1 = e0 + 1u;
+    } else {
+      const uint32_t h = (y * kPrime1) ^ (z * kPrime2);
+      e0 = ((i0[0] ^ h) & ((1u << kLog2T) - 1u)) - first;
+      e1 = (((i0[0] + 1u) ^ h) & ((1u << kLog2T) - 1u)) - first;
+    }
+    const float wyz = ((c & 1) ? fr[1] : 1.0f - fr[1]) * ((c >> 1) ? fr[2] : 1.0f - fr[2]);
+    const float w0 = (1.0f - fr[0]) * wyz, w1 = fr[0] * wyz;
+    if (e0 < count) {
+      atomicAdd(&acc[2 * e0], (double)(w0 * d0));
+      atomicAdd(&acc[2 * e0 + 1], (double)(w0 * d1));
+    }
+    if (e1 < count) {
+      atomicAdd(&acc[2 * e1], (double)(w1 * d0));
+      atomicAdd(&acc[2 * e1 + 1], (double)(w1 * d1));
+    }
+  }
+}
+
+// Samples [lo, hi) whose key names this slice, through a per-wavefront queue.
+// A lane reads the keys of 4 consecutive samples at once (keys_l is 16-byte
+// aligned), the next 4 while these are matched.
+template <int L>
+__device__ __forceinline__ void slice_walk(const uint32_t *__restrict__ keys_l,
+                                           const float4 *__restrict__ pos,
+                                           const float4 *__restrict__ df_part, uint32_t lo,
+                                           uint32_t hi, uint32_t slice, uint32_t count,
+                                           double *acc, uint32_t *queue) {
+  constexpr bool kDense = L < kDenseLevels;
+  const uint32_t first = slice * kSliceEntries;
+  const uint32_t want = slice * 0x01010101u;
+  // dense levels: a pair whose x corner lies in this slice (straddle flag or
+  // not), or a pair of the previous slice whose x + 1 corner opens this one
+  const uint32_t id_mask = kDense ? (kKeyStraddle - 1u) * 0x01010101u : ~0u;
+  const uint32_t want2 = kDense && slice ? ((slice - 1u) | kKeyStraddle) * 0x01010101u : want;
+  const int lane = lane_id();
+  const uint64_t below = lanes_below(lane);
+  const uint32_t wave = threadIdx.x >> 6;
+  const uint32_t lo4 = lo & ~3u;
+  constexpr uint32_t kStep = 4 * kSliceThreads;
+  auto load4 = [&](uint32_t s0) -> uint4 {
+    return s0 < hi ? *reinterpret_cast<const uint4 *>(keys_l + s0)
+                   : make_uint4(kKeyNone, kKeyNone, kKeyNone, kKeyNone);
+  };
+  uint32_t head = 0, queued = 0;  // wavefront-uniform
+  uint4 next = load4(lo4 + wave * 256 + 4 * (uint32_t)lane);
+  for (uint32_t base = lo4 + wave * 256; base < hi; base += kStep) {
+    const uint32_t s0 = base + 4 * (uint32_t)lane;
+    const uint4 k4 = next;
+    next = load4(s0 + kStep);
+    const uint32_t kk[4] = {k4.x, k4.y, k4.z, k4.w};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint32_t s = s0 + (uint32_t)j;
+      const uint32_t key = s >= lo && s < hi ? kk[j] : kKeyNone;  // kKeyNone & id_mask: 63, no dense slice
+      const uint32_t t1 = (key & id_mask) ^ want, t2 = key ^ want2;
+      const bool match = (((t1 - 0x01010101u) & ~t1) | ((t2 - 0x01010101u) & ~t2)) & 0x80808080u;
+      const uint64_t m = __ballot(match);
+      if (match) queue[(head + queued + (uint32_t)__popcll(m & below)) & (kSliceQueue - 1u)] = s;
+      queued += (uint32_t)__popcll(m);
+      if (queued >= 64) {
+        wave_lds_fence();
+        const uint32_t q = queue[(head + (uint32_t)lane) & (kSliceQueue - 1u)];
+        wave_lds_fence();
+        head += 64;
+        queued -= 64;
+        slice_sample<L>(pos, df_part, q, first, count, acc);
       }
     }
   }
+  wave_lds_fence();
+  if ((uint32_t)lane < queued)
+    slice_sample<L>(pos, df_part, queue[(head + (uint32_t)lane) & (kSliceQueue - 1u)], first, count, acc);
+}
+
+__global__ __launch_bounds__(kSliceThreads) void scatter_slices(
+    const float4 *__restrict__ pos, const float4 *__restrict__ dfeat,
+    const uint32_t *__restrict__ keys, const uint32_t *__restrict__ n_ptr, uint32_t stride,
+    const SliceTask *__restrict__ tasks, float *grad_table) {
+  __shared__ double acc[2 * kSliceEntries];
+  __shared__ uint32_t queue_all[kSliceWaves][kSliceQueue];
+  const SliceTask tk = tasks[(blockIdx.x % kParts) * kSliceTasksPerXcd + blockIdx.x / kParts];
+  if (tk.level >= (uint32_t)kLevels) return;  // empty slot (block-uniform)
+  const int level = (int)tk.level;
+  const int part = min(level, kLevels - 1 - level);
+  const float4 *__restrict__ df_part = dfeat + (size_t)part * stride;
+  const uint32_t *__restrict__ keys_l = keys + (size_t)level * key_stride(stride);
+  uint32_t *queue = queue_all[threadIdx.x >> 6];
+  float *grad_level = grad_table + 2 * (size_t)kDevLevelOffset[0];
+#pragma unroll
+  for (int l = 1; l < kLevels; ++l)
+    if (level == l) grad_level = grad_table + 2 * (size_t)kDevLevelOffset[l];
+  for (uint32_t i = threadIdx.x; i < 2 * tk.count; i += kSliceThreads) acc[i] = 0.0;
   __syncthreads();
-  if (in_lds)
-    for (uint32_t i = threadIdx.x; i < nfl; i += blockDim.x) {
-      const float v = dense_sum[i];
-      if (v != 0.0f) unsafeAtomicAdd(grad_table + off + i, v);
-    }
+  const uint32_t n = min(*n_ptr, stride);
+  const uint32_t lo = (uint32_t)((uint64_t)n * tk.split / tk.n_split);
+  const uint32_t hi = (uint32_t)((uint64_t)n * (tk.split + 1) / tk.n_split);
+#define NES_SLICE_CASE(L)                                                                    \
+  case L:                                                                                    \
+    slice_walk<L>(keys_l, pos, df_part, lo, hi, tk.slice, tk.count, acc, queue);             \
+    break;
+  switch (level) {
+    NES_SLICE_CASE(0) NES_SLICE_CASE(1) NES_SLICE_CASE(2) NES_SLICE_CASE(3)
+    NES_SLICE_CASE(4) NES_SLICE_CASE(5) NES_SLICE_CASE(6) NES_SLICE_CASE(7)
+    NES_SLICE_CASE(8) NES_SLICE_CASE(9) NES_SLICE_CASE(10) NES_SLICE_CASE(11)
+    NES_SLICE_CASE(12) NES_SLICE_CASE(13) NES_SLICE_CASE(14) NES_SLICE_CASE(15)
+  }
+#undef NES_SLICE_CASE
+  __syncthreads();
+  float *g = grad_level + 2 * (size_t)tk.slice * kSliceEntries;
+  for (uint32_t i = threadIdx.x; i < 2 * tk.count; i += kSliceThreads) {
+    const float v = (float)acc[i];
+    if (v == 0.0f) continue;
+    if (tk.n_split == 1)
+      g[i] += v;  // the only writer of this entry
+    else
+      unsafeAtomicAdd(g + i, v);
+  }
 }
 
 __global__ void adam_kernel(float *p, float *grad, float *m, float *v, size_t n, float lr,
@@ -604,10 +799,11 @@ struct nes_trainer {
   size_t n_table = (size_t)kDevTableEntries * 2, n_params = 0;  // device layout
   float *params = nullptr, *grads = nullptr, *m = nullptr, *v = nullptr;
   uint32_t *occ = nullptr;
+  SliceTask *tasks = nullptr;  // kParts * kSliceTasksPerXcd
   ViewDesc *poses = nullptr;
   float *targets = nullptr;
-  float *target = nullptr, *loss = nullptr;
-  DevBuf dout, dfeat, xo, xd, dCb;
+  float *target = nullptr, *loss = nullptr, *loss_part = nullptr;
+  DevBuf dout, dfeat, keys, xo, xd, dCb;
   float *dC = nullptr;
   RenderWork work;
   hipStream_t stream = nullptr;
@@ -640,6 +836,7 @@ static int trainer_grow(nes_trainer *t, size_t cap) {
   NES_TRY(render_reserve_samples(t->work, NES_PRECISION_FP32, cap));
   NES_TRY(t->dout.reserve(sizeof(float4) * (cap + 1)));
   NES_TRY(t->dfeat.reserve(sizeof(float4) * kParts * (cap + 1)));
+  NES_TRY(t->keys.reserve(sizeof(uint32_t) * kLevels * key_stride((uint32_t)cap)));
   return NES_OK;
 }
 
@@ -691,9 +888,11 @@ static int trainer_backward(nes_trainer *t, uint32_t n_rays) {
   ma.grad_w = t->grads + t->n_table;
   mlp_train<true><<<256, 256, 0, s>>>(ma);
   NES_HIP_TRY(hipGetLastError());
-  const uint32_t bpp = kScatterBlocks;
-  scatter_parts<<<bpp * kParts, 256, 0, s>>>(w.sample_pos.as<float4>(), t->dfeat.as<float4>(),
-                                             w.total.as<uint32_t>(), w.capacity, t->grads, bpp);
+  slice_keys<<<1024, 256, 0, s>>>(w.sample_pos.as<float4>(), w.total.as<uint32_t>(), w.capacity,
+                                  t->keys.as<uint32_t>());
+  scatter_slices<<<kParts * kSliceTasksPerXcd, kSliceThreads, 0, s>>>(
+      w.sample_pos.as<float4>(), t->dfeat.as<float4>(), t->keys.as<uint32_t>(),
+      w.total.as<uint32_t>(), w.capacity, t->tasks, t->grads);
   NES_HIP_TRY(hipGetLastError());
   return NES_OK;
 }
@@ -718,10 +917,11 @@ static int trainer_batch(nes_trainer *t, uint64_t step, float *loss_out) {
   hipStream_t s = t->stream;
   NES_TRY(trainer_prepare(t, n));
   NES_TRY(trainer_forward(t, n, step_rays(t, step)));
-  zero_f32<<<1, 64, 0, s>>>(t->loss, 1);
   const OutLayout L = out_layout(t->work.out_px);
   const float *rgbf = reinterpret_cast<const float *>(t->work.out.as<uint8_t>() + L.rgbf);
-  train_loss<<<(n + 255) / 256, 256, 0, s>>>(rgbf, t->target, n, t->dC, t->loss);
+  const uint32_t n_part = (n + kLossThreads - 1) / kLossThreads;
+  train_loss<<<n_part, kLossThreads, 0, s>>>(rgbf, t->target, n, t->dC, t->loss_part);
+  loss_sum<<<1, kLossThreads, 0, s>>>(t->loss_part, n_part, t->loss);
   NES_HIP_TRY(hipGetLastError());
   NES_TRY(trainer_backward(t, n));
   if (loss_out) {
@@ -757,7 +957,16 @@ int nes_trainer_create(const nes_scene_desc *init, const nes_train_config *cfg,
     for (float **p : {&t->params, &t->grads, &t->m, &t->v})
       NES_HIP_TRY(hipMalloc(p, t->n_params * sizeof(float)));
     NES_HIP_TRY(hipMalloc(&t->occ, NES_OCC_WORDS * sizeof(uint32_t)));
+    const std::vector<SliceTask> tasks = slice_tasks();
+    if (tasks.empty()) {
+      set_last_error("gradient slices exceed the workgroups per XCD");
+      return NES_EIO;
+    }
+    NES_HIP_TRY(hipMalloc(&t->tasks, tasks.size() * sizeof(SliceTask)));
+    NES_HIP_TRY(hipMemcpy(t->tasks, tasks.data(), tasks.size() * sizeof(SliceTask),
+                          hipMemcpyHostToDevice));
     NES_HIP_TRY(hipMalloc(&t->loss, 64));
+    NES_HIP_TRY(hipMalloc(&t->loss_part, (kMaxSlots / kLossThreads) * sizeof(float)));
     const size_t nr = t->cfg.rays_per_batch;
     NES_HIP_TRY(hipMalloc(&t->target, nr * 3 * sizeof(float)));
     NES_TRY(t->dCb.reserve(nr * 3 * sizeof(float)));
@@ -803,12 +1012,14 @@ int nes_trainer_destroy(nes_trainer *t) {
   if (!t) return NES_EINVAL;
   if (t->stream) hipStreamSynchronize(t->stream);
   t->dCb.release();
-  for (float *p : {t->params, t->grads, t->m, t->v, t->targets, t->target, t->loss})
+  for (float *p : {t->params, t->grads, t->m, t->v, t->targets, t->target, t->loss, t->loss_part})
     if (p) hipFree(p);
   if (t->occ) hipFree(t->occ);
+  if (t->tasks) hipFree(t->tasks);
   if (t->poses) hipFree(t->poses);
   t->dout.release();
   t->dfeat.release();
+  t->keys.release();
   t->xo.release();
   t->xd.release();
   render_work_free(t->work);
