@@ -286,7 +286,10 @@ __global__ __launch_bounds__(256) void mlp_train(TrainMlpArgs a) {
     wgrad(IC<4>{}, IC<2>{}, IC<32>{});
   }
   if (kBwd) {
-    // this wave's sums into the workgroup's LDS dW, dest(o, k) = dW index or -1
+    // this wave's sums into the workgroup's LDS dW, dest(o, k) = dW index or
+    // -1.  Within a wave every (o, k) is a different index, so the waves add
+    // in turn with plain loads and stores (an FP32 atomic add in LDS costs
+    // about 3 cycles per lane).
     auto flush = [&](auto ob, auto ib, auto base, auto dest) {
       constexpr int OB = decltype(ob)::value, IB = decltype(ib)::value, BASE = decltype(base)::value;
 #pragma unroll
@@ -296,18 +299,22 @@ __global__ __launch_bounds__(256) void mlp_train(TrainMlpArgs a) {
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             const int idx = dest(16 * mo + 4 * g + r, 16 * mk + j);
-            if (idx >= 0) atomicAdd(&dW[idx], wacc[BASE + mo * IB + mk][r]);
+            if (idx >= 0) dW[idx] += wacc[BASE + mo * IB + mk][r];
           }
     };
-    flush(IC<1>{}, IC<4>{}, IC<0>{}, [](int o, int k) { return o < 3 ? kW5 + o * 64 + k : -1; });
-    flush(IC<4>{}, IC<4>{}, IC<4>{}, [](int o, int k) { return kW4 + o * 64 + k; });
-    flush(IC<4>{}, IC<2>{}, IC<20>{}, [](int o, int k) {
-      const int col = w3_col(k);
-      return col < 0 ? -1 : kW3 + o * 31 + col;
-    });
-    flush(IC<1>{}, IC<4>{}, IC<28>{}, [](int o, int k) { return kW2 + o * 64 + k; });
-    flush(IC<4>{}, IC<2>{}, IC<32>{}, [](int o, int k) { return kW1 + o * 32 + w1_col(k); });
-    __syncthreads();
+    for (int turn = 0; turn < kTrainWaves; ++turn) {
+      if (wave == turn) {
+        flush(IC<1>{}, IC<4>{}, IC<0>{}, [](int o, int k) { return o < 3 ? kW5 + o * 64 + k : -1; });
+        flush(IC<4>{}, IC<4>{}, IC<4>{}, [](int o, int k) { return kW4 + o * 64 + k; });
+        flush(IC<4>{}, IC<2>{}, IC<20>{}, [](int o, int k) {
+          const int col = w3_col(k);
+          return col < 0 ? -1 : kW3 + o * 31 + col;
+        });
+        flush(IC<1>{}, IC<4>{}, IC<28>{}, [](int o, int k) { return kW2 + o * 64 + k; });
+        flush(IC<4>{}, IC<2>{}, IC<32>{}, [](int o, int k) { return kW1 + o * 32 + w1_col(k); });
+      }
+      __syncthreads();
+    }
     for (int i = (int)threadIdx.x; i < kWeights; i += (int)blockDim.x)
       if (dW[i] != 0.0f) unsafeAtomicAdd(a.grad_w + i, dW[i]);
   }
@@ -510,48 +517,52 @@ __global__ __launch_bounds__(256) void train_dcomposite(
 // (measured with every lane of 256 workgroups adding at random addresses),
 // and the wider sums lose nothing to the order of the adds.
 //
-// slice_keys first stores, per sample and level, the slice of each of the
-// four (y, z) corner pairs, one byte each (hashed levels: bits 13-18 of
-// y P1 ^ z P2, which x < 2^13 leaves alone; dense levels: the slice of the
-// pair's x corner, with kKeyStraddle when its x + 1 corner opens the next
-// slice).  A slice's workgroup then reads 4 bytes per sample, tests them
-// against its slice with a SWAR byte match, queues the matching samples per
-// wavefront in LDS and processes them 64 at a time: only those load their
-// position and feature gradient.
+// The samples are first sorted into one bucket per (level, slice) that their
+// corners touch (a counting sort: bucket_count, bucket_scan, bucket_place),
+// so each slice's workgroup reads only the samples it needs.  A hashed
+// level's corner pair (y, z) lies in slice (y P1 ^ z P2) >> 13 (x < 2^13
+// leaves those bits alone); a dense level's pair in the slices of its x and
+// x + 1 corners.
 //
-// The hashed levels (64 slices each) walk all samples and own their entries
-// outright, so their final add is a plain load and store; the dense levels,
-// whose few slices every sample hits, split the samples over kDenseGroups
-// workgroups per slice, each adding its partial slice with global atomics.
+// The hashed levels' slices (64 each) take their whole bucket and own their
+// entries outright, so their final add is a plain load and store; the dense
+// levels, whose few slices every sample hits, split their buckets over
+// kDenseGroups workgroups, each adding its partial slice with global atomics.
 // Workgroup b runs on XCD b % 8; a hashed slice runs on the XCD named by its
 // level's part (levels p and 15 - p on XCD p), whose L2 holds that feature
 // plane, and the lists are then evened out from their tails.
 constexpr uint32_t kSliceEntries = 8192;  // 128 KB of FP64 feature pairs
 constexpr int kSliceShift = 13;
 constexpr int kSliceThreads = 1024;
-constexpr int kSliceWaves = kSliceThreads / 64;
-constexpr uint32_t kSliceQueue = 256;  // queued samples per wavefront (a ring)
+constexpr int kSliceUnroll = 8;  // bucket entries per thread and loop iteration
 constexpr uint32_t kSliceTasksPerXcd = 128;
-constexpr uint32_t kKeyStraddle = 0x40u;
-constexpr uint32_t kKeyNone = 0xffffffffu;  // matches no slice
 constexpr uint32_t kDenseGroups[kDenseLevels] = {32, 16, 8, 4, 2};
 static_assert(kSliceEntries == 1u << kSliceShift, "slice size");
-static_assert(((kLevelOffset[kDenseLevels] - kLevelOffset[kDenseLevels - 1]) >> kSliceShift) <
-                  kKeyStraddle - 1u,
-              "dense slice ids fit below the straddle flag, and below kKeyNone's masked byte");
 static_assert(kLevelRes[kLevels - 1] < (int)kSliceEntries,
               "x and x + 1 differ below the slice bits of a hash");
-static_assert(kSliceQueue >= 128 && (kSliceQueue & (kSliceQueue - 1)) == 0,
-              "a ring of at least 63 waiting + 64 new samples");
 
-// Keys of one level: a 16-byte aligned row per level.
-__host__ __device__ constexpr uint32_t key_stride(uint32_t stride) { return (stride + 3u) & ~3u; }
+__host__ __device__ constexpr uint32_t level_slices(int l) {
+  return (kLevelOffset[l + 1] - kLevelOffset[l] + kSliceEntries - 1u) >> kSliceShift;
+}
+__host__ __device__ constexpr uint32_t bucket_base(int l) {
+  return l == 0 ? 0u : bucket_base(l - 1) + level_slices(l - 1);
+}
+constexpr uint32_t kBuckets = bucket_base(kLevels);  // 747
+constexpr int kBucketThreads = 256;
+constexpr int kBucketBlocks = 512;
+constexpr int kBucketScanThreads = 1024;
+static_assert(kBuckets <= (uint32_t)kBucketScanThreads, "one bucket per scan thread");
+static_assert((kLevelRes[kDenseLevels - 1] + 1) * (kLevelRes[kDenseLevels - 1] + 2) + 1 < (int)kSliceEntries,
+              "a dense level's corners span at most two slices");
+// bucket entries per sample: four per hashed level, at most two per dense one
+constexpr uint32_t kEntriesPerSample = 4 * (kLevels - kDenseLevels) + 2 * kDenseLevels;
+constexpr uint32_t kMaxBucketSamples = 1u << 30;  // entries hold sample << 2
 
 struct SliceTask {
   uint32_t level;           // kLevels or more marks an empty slot
   uint32_t slice;           // entries [slice * kSliceEntries, + count) of the level
   uint32_t count;
-  uint32_t split, n_split;  // samples [n * split / n_split, n * (split + 1) / n_split)
+  uint32_t split, n_split;  // entries [c * split / n_split, c * (split + 1) / n_split) of the bucket
 };
 
 std::vector<SliceTask> slice_tasks() {
@@ -560,7 +571,7 @@ std::vector<SliceTask> slice_tasks() {
     const uint32_t size = kLevelOffset[l + 1] - kLevelOffset[l];
     const uint32_t groups = l < kDenseLevels ? kDenseGroups[l] : 1u;
     const int home = min(l, kLevels - 1 - l);
-    for (uint32_t k = 0; k * kSliceEntries < size; ++k)
+    for (uint32_t k = 0; k < level_slices(l); ++k)
       for (uint32_t g = 0; g < groups; ++g)
         xcd[home].push_back(SliceTask{(uint32_t)l, k, std::min(kSliceEntries, size - k * kSliceEntries),
                                       g, groups});
@@ -585,10 +596,14 @@ std::vector<SliceTask> slice_tasks() {
   return t;
 }
 
-// Per sample, per level: the slice of each (y, z) corner pair, one byte each.
-__global__ void slice_keys(const float4 *__restrict__ pos, const uint32_t *__restrict__ n_ptr,
-                           uint32_t stride, uint32_t *__restrict__ keys) {
-  const uint32_t n = min(*n_ptr, stride);
+// Every bucket entry of the workgroup's samples, in a fixed order: f(bucket,
+// entry).  An entry is sample << 2 | pair: a hashed level's four (y, z)
+// corner pairs enter the buckets of their slices separately; a dense level's
+// pairs span at most two neighbouring slices (the x corner of pair 0 is the
+// smallest entry, the x + 1 corner of pair 3 the largest, fewer than
+// kSliceEntries apart), and the sample enters each of them once, pair 0.
+template <class F>
+__device__ __forceinline__ void for_each_bucket(const float4 *__restrict__ pos, uint32_t n, F &&f) {
   for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
     const float4 p = pos[s];
 #pragma unroll
@@ -596,44 +611,93 @@ __global__ void slice_keys(const float4 *__restrict__ pos, const uint32_t *__res
       const float scale = (float)kLevelRes[l];
       const uint32_t y = (uint32_t)fminf(floorf(p.y * scale), scale - 1.0f);
       const uint32_t z = (uint32_t)fminf(floorf(p.z * scale), scale - 1.0f);
-      uint32_t key = 0;
       if (l < kDenseLevels) {
         const uint32_t n1 = (uint32_t)kLevelRes[l] + 1u;
         const uint32_t x = (uint32_t)fminf(floorf(p.x * scale), scale - 1.0f);
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          const uint32_t e = x + (y + (c & 1)) * n1 + (z + (c >> 1)) * n1 * n1;
-          const uint32_t b = (e >> kSliceShift) |
-                             ((e & (kSliceEntries - 1u)) == kSliceEntries - 1u ? kKeyStraddle : 0u);
-          key |= b << (8 * c);
-        }
+        const uint32_t e = x + y * n1 + z * n1 * n1;
+        const uint32_t lo = e >> kSliceShift, hi = (e + n1 * n1 + n1 + 1u) >> kSliceShift;
+        f(bucket_base(l) + lo, s << 2);
+        if (hi != lo) f(bucket_base(l) + hi, s << 2);
       } else {
         const uint32_t hy0 = y * kPrime1, hz0 = z * kPrime2;
         const uint32_t hy[2] = {hy0, hy0 + kPrime1}, hz[2] = {hz0, hz0 + kPrime2};
 #pragma unroll
         for (int c = 0; c < 4; ++c)
-          key |= (((hy[c & 1] ^ hz[c >> 1]) >> kSliceShift) & ((1u << (kLog2T - kSliceShift)) - 1u))
-                 << (8 * c);
+          f(bucket_base(l) + (((hy[c & 1] ^ hz[c >> 1]) & ((1u << kLog2T) - 1u)) >> kSliceShift),
+            (s << 2) | (uint32_t)c);
       }
-      keys[(size_t)l * key_stride(stride) + s] = key;
     }
   }
 }
 
-// One queued sample of level L into the LDS slice (entries [first, first +
-// count) of the level): the corners of its pairs that fall in the slice.
+// Bucket sizes: per workgroup in LDS, then into count[kBuckets] (zeroed before).
+__global__ __launch_bounds__(kBucketThreads) void bucket_count(const float4 *__restrict__ pos,
+                                                              const uint32_t *__restrict__ n_ptr,
+                                                              uint32_t stride, uint32_t *count) {
+  __shared__ uint32_t hist[kBuckets];
+  for (uint32_t b = threadIdx.x; b < kBuckets; b += blockDim.x) hist[b] = 0;
+  __syncthreads();
+  for_each_bucket(pos, min(*n_ptr, stride), [&](uint32_t b, uint32_t) { atomicAdd(&hist[b], 1u); });
+  __syncthreads();
+  for (uint32_t b = threadIdx.x; b < kBuckets; b += blockDim.x)
+    if (hist[b]) atomicAdd(&count[b], hist[b]);
+}
+
+// Exclusive prefix sums of the bucket sizes: off[kBuckets + 1], and the
+// placement cursor of every bucket set to its start.
+__global__ __launch_bounds__(kBucketScanThreads) void bucket_scan(const uint32_t *count,
+                                                                 uint32_t *off, uint32_t *cursor) {
+  __shared__ uint32_t wsum[kBucketScanThreads / 64];
+  const uint32_t b = threadIdx.x;
+  const uint32_t c = b < kBuckets ? count[b] : 0u;
+  const int lane = lane_id();
+  uint32_t inc = c;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t o = __shfl_up(inc, d);
+    if (lane >= d) inc += o;
+  }
+  if (lane == 63) wsum[b >> 6] = inc;
+  __syncthreads();
+  uint32_t ex = inc - c;
+  for (uint32_t w = 0; w < (b >> 6); ++w) ex += wsum[w];
+  if (b < kBuckets) {
+    off[b] = ex;
+    cursor[b] = ex;
+  }
+  if (b == kBuckets - 1) off[kBuckets] = ex + c;
+}
+
+// The samples into their buckets: each workgroup counts its entries again,
+// reserves a range of every bucket with one global atomic, then writes.
+__global__ __launch_bounds__(kBucketThreads) void bucket_place(const float4 *__restrict__ pos,
+                                                              const uint32_t *__restrict__ n_ptr,
+                                                              uint32_t stride, uint32_t *cursor,
+                                                              uint32_t *__restrict__ items) {
+  __shared__ uint32_t hist[kBuckets];
+  for (uint32_t b = threadIdx.x; b < kBuckets; b += blockDim.x) hist[b] = 0;
+  __syncthreads();
+  const uint32_t n = min(*n_ptr, stride);
+  for_each_bucket(pos, n, [&](uint32_t b, uint32_t) { atomicAdd(&hist[b], 1u); });
+  __syncthreads();
+  for (uint32_t b = threadIdx.x; b < kBuckets; b += blockDim.x)
+    hist[b] = hist[b] ? atomicAdd(&cursor[b], hist[b]) : 0u;
+  __syncthreads();
+  for_each_bucket(pos, n, [&](uint32_t b, uint32_t e) { items[atomicAdd(&hist[b], 1u)] = e; });
+}
+
+// One bucket entry of level L (its sample's feature gradient df and position
+// p; pair c on a hashed level, every pair on a dense one) into the LDS slice,
+// entries [first, first + count) of the level: the corners that fall in it.
 template <int L>
-__device__ __forceinline__ void slice_sample(const float4 *__restrict__ pos,
-                                             const float4 *__restrict__ df_part, uint32_t s,
-                                             uint32_t first, uint32_t count, double *acc) {
+__device__ __forceinline__ void slice_entry(const float4 &df, const float4 &p, uint32_t c,
+                                            uint32_t first, uint32_t count, double *acc) {
   constexpr float kScale = (float)kLevelRes[L];
   constexpr bool kDense = L < kDenseLevels;
   constexpr bool kHi = L >= kParts;  // second level of its part: (z, w) of the plane
   constexpr uint32_t kN1 = (uint32_t)kLevelRes[L] + 1u;
-  const float4 df = df_part[s];
   const float d0 = kHi ? df.z : df.x, d1 = kHi ? df.w : df.y;
   if (d0 == 0.0f && d1 == 0.0f) return;
-  const float4 p = pos[s];
   const float xc[3] = {p.x, p.y, p.z};
   uint32_t i0[3];
   float fr[3];
@@ -644,9 +708,8 @@ __device__ __forceinline__ void slice_sample(const float4 *__restrict__ pos,
     i0[ax] = (uint32_t)fl;
     fr[ax] = pp - fl;
   }
-#pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    const uint32_t y = i0[1] + (c & 1), z = i0[2] + (c >> 1);
+  auto add_pair = [&](uint32_t cy, uint32_t cz) {
+    const uint32_t y = i0[1] + cy, z = i0[2] + cz;
     uint32_t e0, e1;
     if (kDense) {
       e0 = i0[0] + y * kN1 + z * (kN1 * kN1) - first;  // wraps above count outside the slice
@@ -656,7 +719,7 @@ __device__ __forceinline__ void slice_sample(const float4 *__restrict__ pos,
       e0 = ((i0[0] ^ h) & ((1u << kLog2T) - 1u)) - first;
       e1 = (((i0[0] + 1u) ^ h) & ((1u << kLog2T) - 1u)) - first;
     }
-    const float wyz = ((c & 1) ? fr[1] : 1.0f - fr[1]) * ((c >> 1) ? fr[2] : 1.0f - fr[2]);
+    const float wyz = (cy ? fr[1] : 1.0f - fr[1]) * (cz ? fr[2] : 1.0f - fr[2]);
     const float w0 = (1.0f - fr[0]) * wyz, w1 = fr[0] * wyz;
     if (e0 < count) {
       atomicAdd(&acc[2 * e0], (double)(w0 * d0));
@@ -666,90 +729,64 @@ __device__ __forceinline__ void slice_sample(const float4 *__restrict__ pos,
       atomicAdd(&acc[2 * e1], (double)(w1 * d0));
       atomicAdd(&acc[2 * e1 + 1], (double)(w1 * d1));
     }
+  };
+  if (kDense) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) add_pair(k & 1, k >> 1);
+  } else {
+    add_pair(c & 1u, c >> 1);
   }
 }
 
-// Samples [lo, hi) whose key names this slice, through a per-wavefront queue.
-// A lane reads the keys of 4 consecutive samples at once (keys_l is 16-byte
-// aligned), the next 4 while these are matched.
+// Bucket entries [lo, hi): kSliceUnroll samples per thread in flight at once.
 template <int L>
-__device__ __forceinline__ void slice_walk(const uint32_t *__restrict__ keys_l,
+__device__ __forceinline__ void slice_walk(const uint32_t *__restrict__ items,
                                            const float4 *__restrict__ pos,
                                            const float4 *__restrict__ df_part, uint32_t lo,
-                                           uint32_t hi, uint32_t slice, uint32_t count,
-                                           double *acc, uint32_t *queue) {
-  constexpr bool kDense = L < kDenseLevels;
-  const uint32_t first = slice * kSliceEntries;
-  const uint32_t want = slice * 0x01010101u;
-  // dense levels: a pair whose x corner lies in this slice (straddle flag or
-  // not), or a pair of the previous slice whose x + 1 corner opens this one
-  const uint32_t id_mask = kDense ? (kKeyStraddle - 1u) * 0x01010101u : ~0u;
-  const uint32_t want2 = kDense && slice ? ((slice - 1u) | kKeyStraddle) * 0x01010101u : want;
-  const int lane = lane_id();
-  const uint64_t below = lanes_below(lane);
-  const uint32_t wave = threadIdx.x >> 6;
-  const uint32_t lo4 = lo & ~3u;
-  constexpr uint32_t kStep = 4 * kSliceThreads;
-  auto load4 = [&](uint32_t s0) -> uint4 {
-    return s0 < hi ? *reinterpret_cast<const uint4 *>(keys_l + s0)
-                   : make_uint4(kKeyNone, kKeyNone, kKeyNone, kKeyNone);
-  };
-  uint32_t head = 0, queued = 0;  // wavefront-uniform
-  uint4 next = load4(lo4 + wave * 256 + 4 * (uint32_t)lane);
-  for (uint32_t base = lo4 + wave * 256; base < hi; base += kStep) {
-    const uint32_t s0 = base + 4 * (uint32_t)lane;
-    const uint4 k4 = next;
-    next = load4(s0 + kStep);
-    const uint32_t kk[4] = {k4.x, k4.y, k4.z, k4.w};
+                                           uint32_t hi, uint32_t first, uint32_t count,
+                                           double *acc) {
+  for (uint32_t base = lo; base < hi; base += kSliceThreads * kSliceUnroll) {
+    float4 df[kSliceUnroll], p[kSliceUnroll];
+    uint32_t ent[kSliceUnroll];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const uint32_t s = s0 + (uint32_t)j;
-      const uint32_t key = s >= lo && s < hi ? kk[j] : kKeyNone;  // kKeyNone & id_mask: 63, no dense slice
-      const uint32_t t1 = (key & id_mask) ^ want, t2 = key ^ want2;
-      const bool match = (((t1 - 0x01010101u) & ~t1) | ((t2 - 0x01010101u) & ~t2)) & 0x80808080u;
-      const uint64_t m = __ballot(match);
-      if (match) queue[(head + queued + (uint32_t)__popcll(m & below)) & (kSliceQueue - 1u)] = s;
-      queued += (uint32_t)__popcll(m);
-      if (queued >= 64) {
-        wave_lds_fence();
-        const uint32_t q = queue[(head + (uint32_t)lane) & (kSliceQueue - 1u)];
-        wave_lds_fence();
-        head += 64;
-        queued -= 64;
-        slice_sample<L>(pos, df_part, q, first, count, acc);
-      }
+    for (int u = 0; u < kSliceUnroll; ++u) {
+      const uint32_t i = base + u * kSliceThreads + threadIdx.x;
+      ent[u] = i < hi ? items[i] : 0u;
+      df[u] = i < hi ? df_part[ent[u] >> 2] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      p[u] = i < hi ? pos[ent[u] >> 2] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
+#pragma unroll
+    for (int u = 0; u < kSliceUnroll; ++u) slice_entry<L>(df[u], p[u], ent[u] & 3u, first, count, acc);
   }
-  wave_lds_fence();
-  if ((uint32_t)lane < queued)
-    slice_sample<L>(pos, df_part, queue[(head + (uint32_t)lane) & (kSliceQueue - 1u)], first, count, acc);
 }
 
 __global__ __launch_bounds__(kSliceThreads) void scatter_slices(
     const float4 *__restrict__ pos, const float4 *__restrict__ dfeat,
-    const uint32_t *__restrict__ keys, const uint32_t *__restrict__ n_ptr, uint32_t stride,
+    const uint32_t *__restrict__ items, const uint32_t *__restrict__ off, uint32_t stride,
     const SliceTask *__restrict__ tasks, float *grad_table) {
   __shared__ double acc[2 * kSliceEntries];
-  __shared__ uint32_t queue_all[kSliceWaves][kSliceQueue];
   const SliceTask tk = tasks[(blockIdx.x % kParts) * kSliceTasksPerXcd + blockIdx.x / kParts];
   if (tk.level >= (uint32_t)kLevels) return;  // empty slot (block-uniform)
   const int level = (int)tk.level;
   const int part = min(level, kLevels - 1 - level);
   const float4 *__restrict__ df_part = dfeat + (size_t)part * stride;
-  const uint32_t *__restrict__ keys_l = keys + (size_t)level * key_stride(stride);
-  uint32_t *queue = queue_all[threadIdx.x >> 6];
   float *grad_level = grad_table + 2 * (size_t)kDevLevelOffset[0];
+  uint32_t bucket = tk.slice;
 #pragma unroll
   for (int l = 1; l < kLevels; ++l)
-    if (level == l) grad_level = grad_table + 2 * (size_t)kDevLevelOffset[l];
+    if (level == l) {
+      grad_level = grad_table + 2 * (size_t)kDevLevelOffset[l];
+      bucket = bucket_base(l) + tk.slice;
+    }
   for (uint32_t i = threadIdx.x; i < 2 * tk.count; i += kSliceThreads) acc[i] = 0.0;
   __syncthreads();
-  const uint32_t n = min(*n_ptr, stride);
-  const uint32_t lo = (uint32_t)((uint64_t)n * tk.split / tk.n_split);
-  const uint32_t hi = (uint32_t)((uint64_t)n * (tk.split + 1) / tk.n_split);
-#define NES_SLICE_CASE(L)                                                                    \
-  case L:                                                                                    \
-    slice_walk<L>(keys_l, pos, df_part, lo, hi, tk.slice, tk.count, acc, queue);             \
+  const uint32_t b0 = off[bucket], cnt = off[bucket + 1] - b0;
+  const uint32_t lo = b0 + (uint32_t)((uint64_t)cnt * tk.split / tk.n_split);
+  const uint32_t hi = b0 + (uint32_t)((uint64_t)cnt * (tk.split + 1) / tk.n_split);
+  const uint32_t first = tk.slice * kSliceEntries;
+#define NES_SLICE_CASE(L)                                                  \
+  case L:                                                                  \
+    slice_walk<L>(items, pos, df_part, lo, hi, first, tk.count, acc);      \
     break;
   switch (level) {
     NES_SLICE_CASE(0) NES_SLICE_CASE(1) NES_SLICE_CASE(2) NES_SLICE_CASE(3)
@@ -759,7 +796,7 @@ __global__ __launch_bounds__(kSliceThreads) void scatter_slices(
   }
 #undef NES_SLICE_CASE
   __syncthreads();
-  float *g = grad_level + 2 * (size_t)tk.slice * kSliceEntries;
+  float *g = grad_level + 2 * (size_t)first;
   for (uint32_t i = threadIdx.x; i < 2 * tk.count; i += kSliceThreads) {
     const float v = (float)acc[i];
     if (v == 0.0f) continue;
@@ -803,7 +840,8 @@ struct nes_trainer {
   ViewDesc *poses = nullptr;
   float *targets = nullptr;
   float *target = nullptr, *loss = nullptr, *loss_part = nullptr;
-  DevBuf dout, dfeat, keys, xo, xd, dCb;
+  DevBuf dout, dfeat, items, xo, xd, dCb;
+  uint32_t *bucket_count = nullptr, *bucket_off = nullptr, *bucket_cursor = nullptr;
   float *dC = nullptr;
   RenderWork work;
   hipStream_t stream = nullptr;
@@ -836,7 +874,11 @@ static int trainer_grow(nes_trainer *t, size_t cap) {
   NES_TRY(render_reserve_samples(t->work, NES_PRECISION_FP32, cap));
   NES_TRY(t->dout.reserve(sizeof(float4) * (cap + 1)));
   NES_TRY(t->dfeat.reserve(sizeof(float4) * kParts * (cap + 1)));
-  NES_TRY(t->keys.reserve(sizeof(uint32_t) * kLevels * key_stride((uint32_t)cap)));
+  if (cap >= kMaxBucketSamples) {
+    set_last_error("training sample count exceeds the gradient buckets' 2^30");
+    return NES_ERANGE;
+  }
+  NES_TRY(t->items.reserve(sizeof(uint32_t) * kEntriesPerSample * (cap + 1)));
   return NES_OK;
 }
 
@@ -888,11 +930,15 @@ static int trainer_backward(nes_trainer *t, uint32_t n_rays) {
   ma.grad_w = t->grads + t->n_table;
   mlp_train<true><<<256, 256, 0, s>>>(ma);
   NES_HIP_TRY(hipGetLastError());
-  slice_keys<<<1024, 256, 0, s>>>(w.sample_pos.as<float4>(), w.total.as<uint32_t>(), w.capacity,
-                                  t->keys.as<uint32_t>());
+  zero_f32<<<1, 1024, 0, s>>>(reinterpret_cast<float *>(t->bucket_count), kBuckets);
+  bucket_count<<<kBucketBlocks, kBucketThreads, 0, s>>>(w.sample_pos.as<float4>(), w.total.as<uint32_t>(),
+                                                        w.capacity, t->bucket_count);
+  bucket_scan<<<1, kBucketScanThreads, 0, s>>>(t->bucket_count, t->bucket_off, t->bucket_cursor);
+  bucket_place<<<kBucketBlocks, kBucketThreads, 0, s>>>(w.sample_pos.as<float4>(), w.total.as<uint32_t>(),
+                                                        w.capacity, t->bucket_cursor, t->items.as<uint32_t>());
   scatter_slices<<<kParts * kSliceTasksPerXcd, kSliceThreads, 0, s>>>(
-      w.sample_pos.as<float4>(), t->dfeat.as<float4>(), t->keys.as<uint32_t>(),
-      w.total.as<uint32_t>(), w.capacity, t->tasks, t->grads);
+      w.sample_pos.as<float4>(), t->dfeat.as<float4>(), t->items.as<uint32_t>(), t->bucket_off,
+      w.capacity, t->tasks, t->grads);
   NES_HIP_TRY(hipGetLastError());
   return NES_OK;
 }
@@ -963,6 +1009,9 @@ int nes_trainer_create(const nes_scene_desc *init, const nes_train_config *cfg,
       return NES_EIO;
     }
     NES_HIP_TRY(hipMalloc(&t->tasks, tasks.size() * sizeof(SliceTask)));
+    NES_HIP_TRY(hipMalloc(&t->bucket_count, kBuckets * sizeof(uint32_t)));
+    NES_HIP_TRY(hipMalloc(&t->bucket_off, (kBuckets + 1) * sizeof(uint32_t)));
+    NES_HIP_TRY(hipMalloc(&t->bucket_cursor, kBuckets * sizeof(uint32_t)));
     NES_HIP_TRY(hipMemcpy(t->tasks, tasks.data(), tasks.size() * sizeof(SliceTask),
                           hipMemcpyHostToDevice));
     NES_HIP_TRY(hipMalloc(&t->loss, 64));
@@ -1019,7 +1068,9 @@ int nes_trainer_destroy(nes_trainer *t) {
   if (t->poses) hipFree(t->poses);
   t->dout.release();
   t->dfeat.release();
-  t->keys.release();
+  t->items.release();
+  for (uint32_t *p : {t->bucket_count, t->bucket_off, t->bucket_cursor})
+    if (p) hipFree(p);
   t->xo.release();
   t->xd.release();
   render_work_free(t->work);
