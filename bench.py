@@ -170,7 +170,10 @@ def ceilings_leg(N, samples, ms_encode, l2_req_per_launch, l1_acc_per_launch=Non
     fetches = 128 * samples / (ms_encode * 1e-3)
     out = {"l1_lane_loads_per_s": l1, "l2_requests_per_s": l2,
            "encode_corner_fetches_per_s": fetches,
-           "encode_fetches_per_l1_lane_slot": round(fetches / l1, 4)}
+           "encode_fetches_per_l1_lane_slot": round(fetches / l1, 4),
+           # LDS atomic adds (every lane, random addresses): why the training
+           # scatter sums its slices in FP64
+           "lds_atomic_adds_per_s": {k: N.lds_atomic_rate(k) for k in ("f32", "f64", "u32")}}
     if l1_acc_per_launch:
         out["encode_l1_accesses_frac"] = round(l1_acc_per_launch / (ms_encode * 1e-3) / l1, 4)
     if l2_req_per_launch:
@@ -261,7 +264,8 @@ def training_leg(N, ctx, occ, steps=12, warmup=3):
     """Differentiable training step: 2^18 rays per batch drawn from 16 seeded
     poses at 640x480, targets rendered by the seeded teacher scene, student
     table U(-1e-4, 1e-4); forward, L2 loss, backward (composite, MLPs on
-    FP32 MFMA, atomic hash scatter) and Adam per step."""
+    FP32 MFMA, table gradient summed per level slice in LDS) and Adam per
+    step."""
     rng = np.random.default_rng(3)
     poses = np.stack([N.orbit_camera(360.0 * rng.random(), 5.0 + 30.0 * rng.random())
                       for _ in range(16)])

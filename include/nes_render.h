@@ -500,6 +500,10 @@ int nes_device_copy(void *dst, const void *src, size_t bytes, float *elapsed_ms)
  * lane-address rate, one inside an XCD's L2 (but not L1) its L2 request
  * rate: lane_loads / elapsed_ms of the timed launch. */
 int nes_device_gather_rate(size_t region_bytes, float *elapsed_ms, uint64_t *lane_loads);
+/* LDS atomic add probe: every lane of one 1024-thread workgroup per CU adds
+ * at random addresses of a 128 KB LDS array; kind 0 FP32, 1 FP64, 2 32-bit
+ * integer.  lane_ops / elapsed_ms of the timed launch is the chip's rate. */
+int nes_device_lds_atomic_rate(int kind, float *elapsed_ms, uint64_t *lane_ops);
 int nes_device_info(char *name, size_t name_len, int *cu_count,
                     uint64_t *hbm_bytes);
 

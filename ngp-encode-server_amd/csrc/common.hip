@@ -111,6 +111,30 @@ __global__ __launch_bounds__(256) void gather_probe(const uint32_t *__restrict__
   if (acc == 0x9E3779B9u) sink[threadIdx.x] = acc;  // keeps the loads live
 }
 
+// LDS atomic add rate: every lane of a 1024-thread workgroup adds at random
+// addresses of a 128 KB LDS array (kind 0: FP32 add, 1: FP64 add, 2: 32-bit
+// integer add), one workgroup per CU.
+constexpr int kLdsProbeThreads = 1024;
+constexpr uint32_t kLdsProbeWords = 32768;  // 128 KB
+template <int KIND>
+__global__ __launch_bounds__(kLdsProbeThreads) void lds_atomic_probe(uint32_t iters, float *sink) {
+  __shared__ double lds[kLdsProbeWords / 2];
+  float *f = reinterpret_cast<float *>(lds);
+  uint32_t *u = reinterpret_cast<uint32_t *>(lds);
+  for (uint32_t i = threadIdx.x; i < kLdsProbeWords / 2; i += kLdsProbeThreads) lds[i] = 0.0;
+  __syncthreads();
+  uint32_t h = (blockIdx.x * kLdsProbeThreads + threadIdx.x + 1u) * 0x9E3779B1u;
+  for (uint32_t i = 0; i < iters; ++i) {
+    h = h * 1664525u + 1013904223u;
+    const uint32_t a = (h >> 10) & (kLdsProbeWords - 1u);
+    if (KIND == 0) atomicAdd(&f[a], 1.0f);
+    if (KIND == 1) atomicAdd(&lds[a >> 1], 1.0);
+    if (KIND == 2) atomicAdd(&u[a], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) sink[blockIdx.x] = f[blockIdx.x & (kLdsProbeWords - 1u)];
+}
+
 }  // namespace nes
 
 extern "C" {
@@ -221,6 +245,32 @@ int nes_device_gather_rate(size_t region_bytes, float *elapsed_ms, uint64_t *lan
   if (e1) hipEventDestroy(e1);
   hipFree(sink);
   hipFree(buf);
+  return rc;
+}
+
+int nes_device_lds_atomic_rate(int kind, float *elapsed_ms, uint64_t *lane_ops) {
+  if (kind < 0 || kind > 2) return NES_EINVAL;
+  NES_TRY(nes::ensure_device());
+  hipStream_t s = nes::library_stream();
+  constexpr uint32_t kBlocks = 256, kIters = 1024;
+  float *sink = nullptr;
+  NES_HIP_TRY(hipMalloc(&sink, kBlocks * sizeof(float)));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = NES_OK;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = NES_EIO;
+  for (int run = 0; rc == NES_OK && run < 2; ++run) {  // the first launch is a warm-up
+    hipEventRecord(e0, s);
+    if (kind == 0) nes::lds_atomic_probe<0><<<kBlocks, nes::kLdsProbeThreads, 0, s>>>(kIters, sink);
+    if (kind == 1) nes::lds_atomic_probe<1><<<kBlocks, nes::kLdsProbeThreads, 0, s>>>(kIters, sink);
+    if (kind == 2) nes::lds_atomic_probe<2><<<kBlocks, nes::kLdsProbeThreads, 0, s>>>(kIters, sink);
+    hipEventRecord(e1, s);
+    if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess) rc = NES_EIO;
+  }
+  if (rc == NES_OK && elapsed_ms) hipEventElapsedTime(elapsed_ms, e0, e1);
+  if (lane_ops) *lane_ops = (uint64_t)kBlocks * nes::kLdsProbeThreads * kIters;
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  hipFree(sink);
   return rc;
 }
 

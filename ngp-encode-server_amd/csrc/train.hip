@@ -64,7 +64,7 @@ struct TrainMlpArgs {
   const float *weights;     // 9344
   float4 *out;              // fwd: (sigma, r, g, b)
   const float4 *dout;       // bwd: (dL/dlogsigma, dL/dpre_r, _g, _b)
-  float4 *dfeat;            // bwd: [8][stride]
+  float4 *dfeat;            // bwd: [8][stride] records of 2 float4: (position, 0), dL/dfeature
   float *grad_w;            // bwd: 9344
 };
 
@@ -100,7 +100,8 @@ __global__ __launch_bounds__(256) void mlp_train(TrainMlpArgs a) {
       const float4 v = a.feat[(size_t)(4 * b + g) * a.stride + sc];
       xf[b][0] = v.x; xf[b][1] = v.y; xf[b][2] = v.z; xf[b][3] = v.w;
     }
-    const RayData ray = load_ray(a.rays, __float_as_uint(a.pos[sc].w));
+    const float4 ps = a.pos[sc];
+    const RayData ray = load_ray(a.rays, __float_as_uint(ps.w));
     float shv[16];
     sh4(ray.d, shv);
     float shq[4];
@@ -217,7 +218,12 @@ __global__ __launch_bounds__(256) void mlp_train(TrainMlpArgs a) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) acc = mfma32(W[kW1 + (16 * b + 4 * g + r) * 32 + col], e1[b][r], acc);
       // rows 16bq + 4g + r = part 4bq + g, component r
-      if (valid) a.dfeat[(size_t)(4 * bq + g) * a.stride + s] = make_float4(acc[0], acc[1], acc[2], acc[3]);
+      // the sample's position beside its gradient: the scatter reads one 32 B record
+      if (valid) {
+        float4 *rec = a.dfeat + 2 * ((size_t)(4 * bq + g) * a.stride + s);
+        rec[0] = make_float4(ps.x, ps.y, ps.z, 0.0f);
+        rec[1] = make_float4(acc[0], acc[1], acc[2], acc[3]);
+      }
     }
     // ---------------- weight gradient: dW_l += sum_s delta_l x_l^T, the
     // tile's delta and x transposed through LDS, accumulated in wacc
@@ -741,8 +747,7 @@ __device__ __forceinline__ void slice_entry(const float4 &df, const float4 &p, u
 // Bucket entries [lo, hi): kSliceUnroll samples per thread in flight at once.
 template <int L>
 __device__ __forceinline__ void slice_walk(const uint32_t *__restrict__ items,
-                                           const float4 *__restrict__ pos,
-                                           const float4 *__restrict__ df_part, uint32_t lo,
+                                           const float4 *__restrict__ rec_part, uint32_t lo,
                                            uint32_t hi, uint32_t first, uint32_t count,
                                            double *acc) {
   for (uint32_t base = lo; base < hi; base += kSliceThreads * kSliceUnroll) {
@@ -752,8 +757,9 @@ __device__ __forceinline__ void slice_walk(const uint32_t *__restrict__ items,
     for (int u = 0; u < kSliceUnroll; ++u) {
       const uint32_t i = base + u * kSliceThreads + threadIdx.x;
       ent[u] = i < hi ? items[i] : 0u;
-      df[u] = i < hi ? df_part[ent[u] >> 2] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      p[u] = i < hi ? pos[ent[u] >> 2] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      const float4 *rec = rec_part + 2 * (size_t)(ent[u] >> 2);  // (position, 0), gradient
+      p[u] = i < hi ? rec[0] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      df[u] = i < hi ? rec[1] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
 #pragma unroll
     for (int u = 0; u < kSliceUnroll; ++u) slice_entry<L>(df[u], p[u], ent[u] & 3u, first, count, acc);
@@ -761,7 +767,7 @@ __device__ __forceinline__ void slice_walk(const uint32_t *__restrict__ items,
 }
 
 __global__ __launch_bounds__(kSliceThreads) void scatter_slices(
-    const float4 *__restrict__ pos, const float4 *__restrict__ dfeat,
+    const float4 *__restrict__ dfeat,
     const uint32_t *__restrict__ items, const uint32_t *__restrict__ off, uint32_t stride,
     const SliceTask *__restrict__ tasks, float *grad_table) {
   __shared__ double acc[2 * kSliceEntries];
@@ -769,7 +775,7 @@ __global__ __launch_bounds__(kSliceThreads) void scatter_slices(
   if (tk.level >= (uint32_t)kLevels) return;  // empty slot (block-uniform)
   const int level = (int)tk.level;
   const int part = min(level, kLevels - 1 - level);
-  const float4 *__restrict__ df_part = dfeat + (size_t)part * stride;
+  const float4 *__restrict__ rec_part = dfeat + 2 * (size_t)part * stride;
   float *grad_level = grad_table + 2 * (size_t)kDevLevelOffset[0];
   uint32_t bucket = tk.slice;
 #pragma unroll
@@ -786,7 +792,7 @@ __global__ __launch_bounds__(kSliceThreads) void scatter_slices(
   const uint32_t first = tk.slice * kSliceEntries;
 #define NES_SLICE_CASE(L)                                                  \
   case L:                                                                  \
-    slice_walk<L>(items, pos, df_part, lo, hi, first, tk.count, acc);      \
+    slice_walk<L>(items, rec_part, lo, hi, first, tk.count, acc);          \
     break;
   switch (level) {
     NES_SLICE_CASE(0) NES_SLICE_CASE(1) NES_SLICE_CASE(2) NES_SLICE_CASE(3)
@@ -873,7 +879,7 @@ static int trainer_prepare(nes_trainer *t, uint32_t n_rays) {
 static int trainer_grow(nes_trainer *t, size_t cap) {
   NES_TRY(render_reserve_samples(t->work, NES_PRECISION_FP32, cap));
   NES_TRY(t->dout.reserve(sizeof(float4) * (cap + 1)));
-  NES_TRY(t->dfeat.reserve(sizeof(float4) * kParts * (cap + 1)));
+  NES_TRY(t->dfeat.reserve(2 * sizeof(float4) * kParts * (cap + 1)));
   if (cap >= kMaxBucketSamples) {
     set_last_error("training sample count exceeds the gradient buckets' 2^30");
     return NES_ERANGE;
@@ -937,7 +943,7 @@ static int trainer_backward(nes_trainer *t, uint32_t n_rays) {
   bucket_place<<<kBucketBlocks, kBucketThreads, 0, s>>>(w.sample_pos.as<float4>(), w.total.as<uint32_t>(),
                                                         w.capacity, t->bucket_cursor, t->items.as<uint32_t>());
   scatter_slices<<<kParts * kSliceTasksPerXcd, kSliceThreads, 0, s>>>(
-      w.sample_pos.as<float4>(), t->dfeat.as<float4>(), t->items.as<uint32_t>(), t->bucket_off,
+      t->dfeat.as<float4>(), t->items.as<uint32_t>(), t->bucket_off,
       w.capacity, t->tasks, t->grads);
   NES_HIP_TRY(hipGetLastError());
   return NES_OK;

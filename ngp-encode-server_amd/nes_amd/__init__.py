@@ -214,6 +214,7 @@ _sig("nes_memcpy_d2h", C.c_int, C.c_void_p, C.c_void_p, C.c_size_t)
 _sig("nes_device_copy", C.c_int, C.c_void_p, C.c_void_p, C.c_size_t, _P(C.c_float))
 _sig("nes_device_info", C.c_int, C.c_char_p, C.c_size_t, _P(C.c_int), _P(C.c_uint64))
 _sig("nes_device_gather_rate", C.c_int, C.c_size_t, _P(C.c_float), _P(C.c_uint64))
+_sig("nes_device_lds_atomic_rate", C.c_int, C.c_int, _P(C.c_float), _P(C.c_uint64))
 
 _lib = None
 
@@ -553,6 +554,19 @@ def gather_rate(region_bytes):
     check(lib().nes_device_gather_rate(region_bytes, C.byref(ms), C.byref(n)),
           "nes_device_gather_rate")
     return n.value / (ms.value * 1e-3), ms.value
+
+
+LDS_ATOMIC_KINDS = {"f32": 0, "f64": 1, "u32": 2}
+
+
+def lds_atomic_rate(kind):
+    """LDS atomic adds per second for the chip (kind "f32", "f64" or "u32"),
+    every lane of one workgroup per CU adding at random addresses."""
+    ms = C.c_float()
+    n = C.c_uint64()
+    check(lib().nes_device_lds_atomic_rate(LDS_ATOMIC_KINDS[kind], C.byref(ms), C.byref(n)),
+          "nes_device_lds_atomic_rate")
+    return n.value / (ms.value * 1e-3)
 
 
 def device_info():
