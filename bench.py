@@ -173,7 +173,7 @@ def ceilings_leg(N, samples, ms_encode, l2_req_per_launch, l1_acc_per_launch=Non
            "encode_fetches_per_l1_lane_slot": round(fetches / l1, 4),
            # LDS atomic adds (every lane, random addresses): why the training
            # scatter sums its slices in FP64
-           "lds_atomic_adds_per_s": {k: N.lds_atomic_rate(k) for k in ("f32", "f64", "u32")}}
+           "lds_atomic_adds_per_s": {k: N.lds_atomic_rate(k) for k in ("f32", "f64", "u32", "f32_cas", "f32x2_cas")}}
     if l1_acc_per_launch:
         out["encode_l1_accesses_frac"] = round(l1_acc_per_launch / (ms_encode * 1e-3) / l1, 4)
     if l2_req_per_launch:

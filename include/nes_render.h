@@ -502,7 +502,9 @@ int nes_device_copy(void *dst, const void *src, size_t bytes, float *elapsed_ms)
 int nes_device_gather_rate(size_t region_bytes, float *elapsed_ms, uint64_t *lane_loads);
 /* LDS atomic add probe: every lane of one 1024-thread workgroup per CU adds
  * at random addresses of a 128 KB LDS array; kind 0 FP32, 1 FP64, 2 32-bit
- * integer.  lane_ops / elapsed_ms of the timed launch is the chip's rate. */
+ * integer, 3 FP32 by compare-and-swap, 4 two FP32 by one 64-bit
+ * compare-and-swap (one lane op per pair).  lane_ops / elapsed_ms of the
+ * timed launch is the chip's rate. */
 int nes_device_lds_atomic_rate(int kind, float *elapsed_ms, uint64_t *lane_ops);
 int nes_device_info(char *name, size_t name_len, int *cu_count,
                     uint64_t *hbm_bytes);

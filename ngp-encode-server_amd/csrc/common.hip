@@ -2,7 +2,7 @@
 // device-memory helpers of the C-ABI.
 #include <cstring>
 
-#include "nes_internal.h"
+#include "device_common.h"
 
 namespace nes {
 
@@ -113,7 +113,8 @@ __global__ __launch_bounds__(256) void gather_probe(const uint32_t *__restrict__
 
 // LDS atomic add rate: every lane of a 1024-thread workgroup adds at random
 // addresses of a 128 KB LDS array (kind 0: FP32 add, 1: FP64 add, 2: 32-bit
-// integer add), one workgroup per CU.
+// integer add, 3: FP32 add by compare-and-swap, 4: two FP32 adds by one
+// 64-bit compare-and-swap), one workgroup per CU.
 constexpr int kLdsProbeThreads = 1024;
 constexpr uint32_t kLdsProbeWords = 32768;  // 128 KB
 template <int KIND>
@@ -130,6 +131,8 @@ __global__ __launch_bounds__(kLdsProbeThreads) void lds_atomic_probe(uint32_t it
     if (KIND == 0) atomicAdd(&f[a], 1.0f);
     if (KIND == 1) atomicAdd(&lds[a >> 1], 1.0);
     if (KIND == 2) atomicAdd(&u[a], 1u);
+    if (KIND == 3) lds_add_f32(&f[a], 1.0f);
+    if (KIND == 4) lds_add_f32x2(&f[a & ~1u], 1.0f, 2.0f);
   }
   __syncthreads();
   if (threadIdx.x == 0) sink[blockIdx.x] = f[blockIdx.x & (kLdsProbeWords - 1u)];
@@ -249,7 +252,7 @@ int nes_device_gather_rate(size_t region_bytes, float *elapsed_ms, uint64_t *lan
 }
 
 int nes_device_lds_atomic_rate(int kind, float *elapsed_ms, uint64_t *lane_ops) {
-  if (kind < 0 || kind > 2) return NES_EINVAL;
+  if (kind < 0 || kind > 4) return NES_EINVAL;
   NES_TRY(nes::ensure_device());
   hipStream_t s = nes::library_stream();
   constexpr uint32_t kBlocks = 256, kIters = 1024;
@@ -263,6 +266,8 @@ int nes_device_lds_atomic_rate(int kind, float *elapsed_ms, uint64_t *lane_ops) 
     if (kind == 0) nes::lds_atomic_probe<0><<<kBlocks, nes::kLdsProbeThreads, 0, s>>>(kIters, sink);
     if (kind == 1) nes::lds_atomic_probe<1><<<kBlocks, nes::kLdsProbeThreads, 0, s>>>(kIters, sink);
     if (kind == 2) nes::lds_atomic_probe<2><<<kBlocks, nes::kLdsProbeThreads, 0, s>>>(kIters, sink);
+    if (kind == 3) nes::lds_atomic_probe<3><<<kBlocks, nes::kLdsProbeThreads, 0, s>>>(kIters, sink);
+    if (kind == 4) nes::lds_atomic_probe<4><<<kBlocks, nes::kLdsProbeThreads, 0, s>>>(kIters, sink);
     hipEventRecord(e1, s);
     if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess) rc = NES_EIO;
   }

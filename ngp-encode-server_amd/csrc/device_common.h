@@ -162,6 +162,36 @@ __device__ __forceinline__ float sigmoidf_(float v) { return 1.0f / (1.0f + expf
 
 // Orders a wavefront's own LDS writes before its later LDS reads of other
 // lanes' data (LDS ops of one wavefront complete in order).
+// FP32 atomic add into LDS as a compare-and-swap loop: the add is IEEE FP32
+// of the value found, retried when another lane changed it in between.  On
+// gfx950 the native FP32 LDS atomic add costs about 3 cycles per lane; a
+// 32-bit compare-and-swap costs a small fraction of that.
+__device__ __forceinline__ void lds_add_f32(float *addr, float v) {
+  uint32_t *w = reinterpret_cast<uint32_t *>(addr);
+  uint32_t seen = *w;
+  for (;;) {
+    const uint32_t prev = atomicCAS(w, seen, __float_as_uint(__uint_as_float(seen) + v));
+    if (prev == seen) break;
+    seen = prev;
+  }
+}
+
+// Both FP32 features of one entry (8-byte aligned) in one 64-bit
+// compare-and-swap loop: one LDS operation where lds_add_f32 needs two.
+__device__ __forceinline__ void lds_add_f32x2(float *addr, float a, float b) {
+  unsigned long long *w = reinterpret_cast<unsigned long long *>(addr);
+  unsigned long long seen = *w;
+  for (;;) {
+    const float lo = __uint_as_float((uint32_t)seen) + a;
+    const float hi = __uint_as_float((uint32_t)(seen >> 32)) + b;
+    const unsigned long long want =
+        (unsigned long long)__float_as_uint(lo) | ((unsigned long long)__float_as_uint(hi) << 32);
+    const unsigned long long prev = atomicCAS(w, seen, want);
+    if (prev == seen) break;
+    seen = prev;
+  }
+}
+
 __device__ __forceinline__ void wave_lds_fence() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();

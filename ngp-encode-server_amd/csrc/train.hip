@@ -517,32 +517,32 @@ __global__ __launch_bounds__(256) void train_dcomposite(
 // global atomics (4 lanes per sample so that the adds of a (y, z) corner pair
 // leave as one request), the 16 levels of a 2^18-ray step took 2.47 ms, 92 %
 // of that rate.  Instead, every workgroup owns kSliceEntries consecutive
-// entries of one level, sums their gradient in LDS and adds the slice into
-// the table gradient once at the end.  The LDS sums are FP64: an FP32 atomic
-// add in LDS costs about 3 cycles per lane on this GPU, an FP64 one 0.34
-// (measured with every lane of 256 workgroups adding at random addresses),
-// and the wider sums lose nothing to the order of the adds.
+// entries of one level, sums their gradient in LDS with FP32 atomic adds and
+// adds the slice into the table gradient once at the end.  The LDS adds are
+// compare-and-swap loops (lds_add_f32): this GPU's native FP32 atomic add in
+// LDS costs about 3 cycles per lane, the loop about a tenth of that
+// (nes_device_lds_atomic_rate: 2.0e11 against 1.8e12 adds per second).
 //
 // The samples are first sorted into one bucket per (level, slice) that their
 // corners touch (a counting sort: bucket_count, bucket_scan, bucket_place),
 // so each slice's workgroup reads only the samples it needs.  A hashed
-// level's corner pair (y, z) lies in slice (y P1 ^ z P2) >> 13 (x < 2^13
-// leaves those bits alone); a dense level's pair in the slices of its x and
-// x + 1 corners.
+// level's corner pair (y, z) lies in slice (y P1 ^ z P2) >> 14 (x < 2^14
+// leaves those bits alone); a dense level's pairs in at most two neighbouring
+// slices.
 //
-// The hashed levels' slices (64 each) take their whole bucket and own their
+// The hashed levels' slices (32 each) take their whole bucket and own their
 // entries outright, so their final add is a plain load and store; the dense
 // levels, whose few slices every sample hits, split their buckets over
 // kDenseGroups workgroups, each adding its partial slice with global atomics.
 // Workgroup b runs on XCD b % 8; a hashed slice runs on the XCD named by its
 // level's part (levels p and 15 - p on XCD p), whose L2 holds that feature
 // plane, and the lists are then evened out from their tails.
-constexpr uint32_t kSliceEntries = 8192;  // 128 KB of FP64 feature pairs
-constexpr int kSliceShift = 13;
+constexpr uint32_t kSliceEntries = 16384;  // 128 KB of FP32 feature pairs
+constexpr int kSliceShift = 14;
 constexpr int kSliceThreads = 1024;
 constexpr int kSliceUnroll = 8;  // bucket entries per thread and loop iteration
 constexpr uint32_t kSliceTasksPerXcd = 128;
-constexpr uint32_t kDenseGroups[kDenseLevels] = {32, 16, 8, 4, 2};
+constexpr uint32_t kDenseGroups[kDenseLevels] = {64, 64, 32, 16, 8};
 static_assert(kSliceEntries == 1u << kSliceShift, "slice size");
 static_assert(kLevelRes[kLevels - 1] < (int)kSliceEntries,
               "x and x + 1 differ below the slice bits of a hash");
@@ -587,10 +587,18 @@ std::vector<SliceTask> slice_tasks() {
     std::stable_sort(v.begin(), v.end(), [](const SliceTask &x, const SliceTask &y) {
       return (x.level >= (uint32_t)kDenseLevels) > (y.level >= (uint32_t)kDenseLevels);
     });
+  // estimated cost: a hashed slice walks its whole bucket, a dense task a
+  // group's share (measured: about twice as long)
+  auto cost = [](const std::vector<SliceTask> &v) {
+    uint32_t c = 0;
+    for (const SliceTask &x : v) c += x.level >= (uint32_t)kDenseLevels ? 2u : 1u;
+    return c;
+  };
   for (;;) {
-    auto lo = std::min_element(xcd.begin(), xcd.end(), [](auto &x, auto &y) { return x.size() < y.size(); });
-    auto hi = std::max_element(xcd.begin(), xcd.end(), [](auto &x, auto &y) { return x.size() < y.size(); });
-    if (hi->size() <= lo->size() + 1) break;
+    auto lo = std::min_element(xcd.begin(), xcd.end(), [&](auto &x, auto &y) { return cost(x) < cost(y); });
+    auto hi = std::max_element(xcd.begin(), xcd.end(), [&](auto &x, auto &y) { return cost(x) < cost(y); });
+    const uint32_t moved = hi->back().level >= (uint32_t)kDenseLevels ? 2u : 1u;
+    if (cost(*hi) < cost(*lo) + 2 * moved) break;
     lo->push_back(hi->back());
     hi->pop_back();
   }
@@ -697,7 +705,7 @@ __global__ __launch_bounds__(kBucketThreads) void bucket_place(const float4 *__r
 // entries [first, first + count) of the level: the corners that fall in it.
 template <int L>
 __device__ __forceinline__ void slice_entry(const float4 &df, const float4 &p, uint32_t c,
-                                            uint32_t first, uint32_t count, double *acc) {
+                                            uint32_t first, uint32_t count, float *acc) {
   constexpr float kScale = (float)kLevelRes[L];
   constexpr bool kDense = L < kDenseLevels;
   constexpr bool kHi = L >= kParts;  // second level of its part: (z, w) of the plane
@@ -727,14 +735,8 @@ __device__ __forceinline__ void slice_entry(const float4 &df, const float4 &p, u
     }
     const float wyz = (cy ? fr[1] : 1.0f - fr[1]) * (cz ? fr[2] : 1.0f - fr[2]);
     const float w0 = (1.0f - fr[0]) * wyz, w1 = fr[0] * wyz;
-    if (e0 < count) {
-      atomicAdd(&acc[2 * e0], (double)(w0 * d0));
-      atomicAdd(&acc[2 * e0 + 1], (double)(w0 * d1));
-    }
-    if (e1 < count) {
-      atomicAdd(&acc[2 * e1], (double)(w1 * d0));
-      atomicAdd(&acc[2 * e1 + 1], (double)(w1 * d1));
-    }
+    if (e0 < count) lds_add_f32x2(&acc[2 * e0], w0 * d0, w0 * d1);
+    if (e1 < count) lds_add_f32x2(&acc[2 * e1], w1 * d0, w1 * d1);
   };
   if (kDense) {
 #pragma unroll
@@ -744,25 +746,67 @@ __device__ __forceinline__ void slice_entry(const float4 &df, const float4 &p, u
   }
 }
 
-// Bucket entries [lo, hi): kSliceUnroll samples per thread in flight at once.
+// A dense level's bucket entry on four lanes (lane bit 1 the x corner, bit 0
+// the feature): one LDS add per corner pair and lane, so one instruction holds
+// the adds of 16 samples, which collide on few entries at these coarse levels.
+template <int L>
+__device__ __forceinline__ void slice_dense_lane(const float4 &df, const float4 &p, uint32_t xb,
+                                                 uint32_t ft, uint32_t first, uint32_t count,
+                                                 float *acc) {
+  constexpr float kScale = (float)kLevelRes[L];
+  constexpr uint32_t kN1 = (uint32_t)kLevelRes[L] + 1u;
+  const float d = ft ? df.y : df.x;  // dense levels are the first level of their part
+  if (df.x == 0.0f && df.y == 0.0f) return;
+  const float xc[3] = {p.x, p.y, p.z};
+  uint32_t i0[3];
+  float fr[3];
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) {
+    const float pp = xc[ax] * kScale;
+    const float fl = fminf(floorf(pp), kScale - 1.0f);
+    i0[ax] = (uint32_t)fl;
+    fr[ax] = pp - fl;
+  }
+  const uint32_t x = i0[0] + xb;
+  const float wx = xb ? fr[0] : 1.0f - fr[0];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const uint32_t y = i0[1] + (c & 1), z = i0[2] + (c >> 1);
+    const uint32_t e = x + y * kN1 + z * (kN1 * kN1) - first;  // wraps above count outside the slice
+    const float w = wx * ((c & 1) ? fr[1] : 1.0f - fr[1]) * ((c >> 1) ? fr[2] : 1.0f - fr[2]);
+    if (e < count) lds_add_f32(&acc[2 * e + ft], w * d);
+  }
+}
+
+// Bucket entries [lo, hi): kSliceUnroll entries per thread (hashed levels) or
+// per four lanes (dense levels) in flight at once.
 template <int L>
 __device__ __forceinline__ void slice_walk(const uint32_t *__restrict__ items,
                                            const float4 *__restrict__ rec_part, uint32_t lo,
                                            uint32_t hi, uint32_t first, uint32_t count,
-                                           double *acc) {
-  for (uint32_t base = lo; base < hi; base += kSliceThreads * kSliceUnroll) {
+                                           float *acc) {
+  constexpr bool kDense = L < kDenseLevels;
+  constexpr uint32_t kLanes = kDense ? 4u : 1u;  // lanes per entry
+  constexpr uint32_t kPerPass = kSliceThreads / kLanes;
+  const uint32_t me = threadIdx.x / kLanes;
+  for (uint32_t base = lo; base < hi; base += kPerPass * kSliceUnroll) {
     float4 df[kSliceUnroll], p[kSliceUnroll];
     uint32_t ent[kSliceUnroll];
 #pragma unroll
     for (int u = 0; u < kSliceUnroll; ++u) {
-      const uint32_t i = base + u * kSliceThreads + threadIdx.x;
+      const uint32_t i = base + u * kPerPass + me;
       ent[u] = i < hi ? items[i] : 0u;
       const float4 *rec = rec_part + 2 * (size_t)(ent[u] >> 2);  // (position, 0), gradient
       p[u] = i < hi ? rec[0] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       df[u] = i < hi ? rec[1] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
 #pragma unroll
-    for (int u = 0; u < kSliceUnroll; ++u) slice_entry<L>(df[u], p[u], ent[u] & 3u, first, count, acc);
+    for (int u = 0; u < kSliceUnroll; ++u) {
+      if (kDense)
+        slice_dense_lane<L>(df[u], p[u], (threadIdx.x >> 1) & 1u, threadIdx.x & 1u, first, count, acc);
+      else
+        slice_entry<L>(df[u], p[u], ent[u] & 3u, first, count, acc);
+    }
   }
 }
 
@@ -770,7 +814,7 @@ __global__ __launch_bounds__(kSliceThreads) void scatter_slices(
     const float4 *__restrict__ dfeat,
     const uint32_t *__restrict__ items, const uint32_t *__restrict__ off, uint32_t stride,
     const SliceTask *__restrict__ tasks, float *grad_table) {
-  __shared__ double acc[2 * kSliceEntries];
+  __shared__ float acc[2 * kSliceEntries];
   const SliceTask tk = tasks[(blockIdx.x % kParts) * kSliceTasksPerXcd + blockIdx.x / kParts];
   if (tk.level >= (uint32_t)kLevels) return;  // empty slot (block-uniform)
   const int level = (int)tk.level;
@@ -784,7 +828,7 @@ __global__ __launch_bounds__(kSliceThreads) void scatter_slices(
       grad_level = grad_table + 2 * (size_t)kDevLevelOffset[l];
       bucket = bucket_base(l) + tk.slice;
     }
-  for (uint32_t i = threadIdx.x; i < 2 * tk.count; i += kSliceThreads) acc[i] = 0.0;
+  for (uint32_t i = threadIdx.x; i < 2 * tk.count; i += kSliceThreads) acc[i] = 0.0f;
   __syncthreads();
   const uint32_t b0 = off[bucket], cnt = off[bucket + 1] - b0;
   const uint32_t lo = b0 + (uint32_t)((uint64_t)cnt * tk.split / tk.n_split);
@@ -804,7 +848,7 @@ __global__ __launch_bounds__(kSliceThreads) void scatter_slices(
   __syncthreads();
   float *g = grad_level + 2 * (size_t)first;
   for (uint32_t i = threadIdx.x; i < 2 * tk.count; i += kSliceThreads) {
-    const float v = (float)acc[i];
+    const float v = acc[i];
     if (v == 0.0f) continue;
     if (tk.n_split == 1)
       g[i] += v;  // the only writer of this entry

@@ -556,11 +556,13 @@ def gather_rate(region_bytes):
     return n.value / (ms.value * 1e-3), ms.value
 
 
-LDS_ATOMIC_KINDS = {"f32": 0, "f64": 1, "u32": 2}
+LDS_ATOMIC_KINDS = {"f32": 0, "f64": 1, "u32": 2, "f32_cas": 3, "f32x2_cas": 4}
 
 
 def lds_atomic_rate(kind):
-    """LDS atomic adds per second for the chip (kind "f32", "f64" or "u32"),
+    """LDS atomic adds per second for the chip (kind "f32", "f64", "u32",
+    "f32_cas", an FP32 add by compare-and-swap, or "f32x2_cas", two FP32
+    adds by one 64-bit compare-and-swap, counted once),
     every lane of one workgroup per CU adding at random addresses."""
     ms = C.c_float()
     n = C.c_uint64()

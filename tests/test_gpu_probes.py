@@ -16,7 +16,7 @@ def test_gather_rate_probe(nes):
     assert L.nes_device_gather_rate(3000, C.byref(out_ms), C.byref(n)) == -22  # not a power of two
 
 
-@pytest.mark.parametrize("kind", ["f32", "f64", "u32"])
+@pytest.mark.parametrize("kind", ["f32", "f64", "u32", "f32_cas", "f32x2_cas"])
 def test_lds_atomic_rate_probe(nes, kind):
     rate = nes.lds_atomic_rate(kind)
     # 256 workgroups x 1024 lanes x 1024 adds; anything from 10 G to 100 T per second
@@ -25,5 +25,5 @@ def test_lds_atomic_rate_probe(nes, kind):
 
 def test_lds_atomic_rate_rejects_unknown_kind(nes):
     out_ms, n = C.c_float(), C.c_uint64()
-    assert nes.lib().nes_device_lds_atomic_rate(3, C.byref(out_ms), C.byref(n)) == -22
+    assert nes.lib().nes_device_lds_atomic_rate(5, C.byref(out_ms), C.byref(n)) == -22
     assert nes.lib().nes_device_lds_atomic_rate(-1, C.byref(out_ms), C.byref(n)) == -22
