@@ -271,7 +271,8 @@ int nes_frameprep_overlay(nes_font *font, const nes_camera *cam,
 /* Differentiable render step for the same scene representation: L2
  * photometric loss, hand-written backward through compositing, the MLPs
  * (FP32 MFMA, activations recomputed per 16-sample tile rather than stored)
- * and an FP32 atomic scatter-add into the hash-table gradient, then Adam.
+ * and an FP32 atomic scatter-add into the hash-table gradient (summed per
+ * table slice in LDS, one slice per workgroup), then Adam.
  * The reference ships no trainer (its renderer loads a trained snapshot);
  * this is the training leg of the render path above. */
 typedef struct nes_trainer nes_trainer;
