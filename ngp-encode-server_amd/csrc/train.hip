@@ -814,7 +814,7 @@ __global__ __launch_bounds__(kSliceThreads) void scatter_slices(
     const float4 *__restrict__ dfeat,
     const uint32_t *__restrict__ items, const uint32_t *__restrict__ off, uint32_t stride,
     const SliceTask *__restrict__ tasks, float *grad_table) {
-  __shared__ float acc[2 * kSliceEntries];
+  __shared__ __align__(16) float acc[2 * kSliceEntries];  // entry pairs for 64-bit compare-and-swap
   const SliceTask tk = tasks[(blockIdx.x % kParts) * kSliceTasksPerXcd + blockIdx.x / kParts];
   if (tk.level >= (uint32_t)kLevels) return;  // empty slot (block-uniform)
   const int level = (int)tk.level;
