@@ -553,7 +553,7 @@ __host__ __device__ constexpr uint32_t level_slices(int l) {
 __host__ __device__ constexpr uint32_t bucket_base(int l) {
   return l == 0 ? 0u : bucket_base(l - 1) + level_slices(l - 1);
 }
-constexpr uint32_t kBuckets = bucket_base(kLevels);  // 747
+constexpr uint32_t kBuckets = bucket_base(kLevels);  // 374
 constexpr int kBucketThreads = 256;
 constexpr int kBucketBlocks = 512;
 constexpr int kBucketScanThreads = 1024;
