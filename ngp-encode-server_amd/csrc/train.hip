@@ -562,7 +562,10 @@ static_assert((kLevelRes[kDenseLevels - 1] + 1) * (kLevelRes[kDenseLevels - 1] +
               "a dense level's corners span at most two slices");
 // bucket entries per sample: four per hashed level, at most two per dense one
 constexpr uint32_t kEntriesPerSample = 4 * (kLevels - kDenseLevels) + 2 * kDenseLevels;
-constexpr uint32_t kMaxBucketSamples = 1u << 30;  // entries hold sample << 2
+// Bucket offsets are 32-bit and entries hold sample << 2: at most this many
+// samples per step (about 79 M; 2^18 rays make about 0.86 M).
+constexpr uint32_t kMaxBucketSamples = 0xffffffffu / kEntriesPerSample;
+static_assert(kMaxBucketSamples < (1u << 30), "entries hold sample << 2");
 
 struct SliceTask {
   uint32_t level;           // kLevels or more marks an empty slot
@@ -924,11 +927,9 @@ static int trainer_grow(nes_trainer *t, size_t cap) {
   NES_TRY(render_reserve_samples(t->work, NES_PRECISION_FP32, cap));
   NES_TRY(t->dout.reserve(sizeof(float4) * (cap + 1)));
   NES_TRY(t->dfeat.reserve(2 * sizeof(float4) * kParts * (cap + 1)));
-  if (cap >= kMaxBucketSamples) {
-    set_last_error("training sample count exceeds the gradient buckets' 2^30");
-    return NES_ERANGE;
-  }
-  NES_TRY(t->items.reserve(sizeof(uint32_t) * kEntriesPerSample * (cap + 1)));
+  // a step's samples stay below kMaxBucketSamples (checked in trainer_forward)
+  NES_TRY(t->items.reserve(sizeof(uint32_t) * kEntriesPerSample *
+                           std::min<size_t>(cap + 1, kMaxBucketSamples)));
   return NES_OK;
 }
 
@@ -945,6 +946,12 @@ static int trainer_forward(nes_trainer *t, uint32_t n_rays, const TrainRayArgs &
     NES_HIP_TRY(hipMemcpyAsync(w.total_host, w.total.ptr, 4, hipMemcpyDeviceToHost, s));
     NES_HIP_TRY(hipStreamSynchronize(s));
     const uint32_t n = w.total_host[0];
+    if (n >= kMaxBucketSamples) {
+      set_last_error("training step has " + std::to_string(n) +
+                     " samples, more than the gradient buckets index (" +
+                     std::to_string(kMaxBucketSamples) + ")");
+      return NES_ERANGE;
+    }
     if (n > w.capacity) {
       NES_TRY(trainer_grow(t, (size_t)n + n / 8 + 1024));
       continue;
