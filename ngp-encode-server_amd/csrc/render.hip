@@ -119,14 +119,25 @@ __global__ __launch_bounds__(256) void march_emit(MarchArgs m) {
       if (m.resume) k = max(k, (int)(counts[slot] >> 16));
       const int kend = min(m.k_end, (int)r.n_steps);
       knext = (uint32_t)max(kend, k);
-      for (; k < kend; ++k) {
-        float x[3], t;
-        step_pos(r, k, x, &t);
-        if (occupied(occ, x)) {
-          bits[k >> 5] |= 1u << (k & 31);
-          if (++cnt == m.cap) {
-            knext = (uint32_t)k + 1u;
-            break;
+      // four steps' occupancy bits in flight at once, then taken in order
+      constexpr int kStepUnroll = 4;
+      bool capped = false;
+      for (; k < kend && !capped; k += kStepUnroll) {
+        bool hit[kStepUnroll];
+#pragma unroll
+        for (int u = 0; u < kStepUnroll; ++u) {
+          float x[3], t;
+          step_pos(r, k + u, x, &t);
+          hit[u] = k + u < kend && occupied(occ, x);
+        }
+#pragma unroll
+        for (int u = 0; u < kStepUnroll; ++u) {
+          if (hit[u] && !capped) {
+            bits[(k + u) >> 5] |= 1u << ((k + u) & 31);
+            if (++cnt == m.cap) {
+              knext = (uint32_t)(k + u) + 1u;
+              capped = true;
+            }
           }
         }
       }
