@@ -105,6 +105,71 @@ def test_autograd_binding(nes, oracle, scene1, trainer, rays):
         assert abs(fd - gw_ref[i]) <= 0.05 * abs(gw_ref[i]), (i, fd, gw_ref[i])
 
 
+SLICE_ENTRIES = 16384  # entries per LDS slice of the table-gradient scatter
+DENSE_RES = [16, 22, 30, 42, 58]
+DENSE_OFFSET = [0, 4913, 17080, 46871, 126378]  # packed level offsets
+
+
+def test_dense_slice_boundaries(nes, oracle, scene1):
+    """A dense level's corner pair whose x corner is the last entry of one
+    LDS slice of the scatter and whose x + 1 corner is the first of the next:
+    each slice's workgroup adds one of them.  Rays through every such cell of
+    levels 2-4 (occupancy only there, so a ray has a few samples and clears
+    the kink margin), gradients against the oracle at those entries and
+    everywhere."""
+    t, w, _ = scene1
+    cells = []
+    for lv in (2, 3, 4):
+        n1 = DENSE_RES[lv] + 1
+        for k in range(1, n1 ** 3 // SLICE_ENTRIES + 1):
+            z, rem = divmod(k * SLICE_ENTRIES - 1, n1 * n1)
+            y, x = divmod(rem, n1)
+            if max(x, y, z) < DENSE_RES[lv]:  # pair (y, z) of cell (x, y, z)
+                cells.append((lv, x, y, z))
+    assert len(cells) >= 10
+    occ_bits = np.zeros(128 ** 3, bool)
+    for lv, x, y, z in cells:
+        lo = [int(np.floor(v / DENSE_RES[lv] * 128)) for v in (x, y, z)]
+        hi = [int(np.ceil((v + 1) / DENSE_RES[lv] * 128)) for v in (x, y, z)]
+        for cz in range(lo[2], hi[2]):
+            for cy in range(lo[1], hi[1]):
+                row = (cz * 128 + cy) * 128
+                occ_bits[row + lo[0]:row + hi[0]] = True
+    occ = np.packbits(occ_bits, bitorder="little").view(np.uint32).copy()
+    rng = np.random.default_rng(11)
+    ro, rd = [], []
+    for lv, x, y, z in cells:
+        for _ in range(6):  # a chord through the cell's middle is longer than a step
+            d = rng.normal(size=3)
+            d /= np.linalg.norm(d)
+            p = (np.array([x, y, z]) + rng.uniform(0.3, 0.7, 3)) / DENSE_RES[lv]
+            ro.append(p - 2.0 * d)
+            rd.append(d)
+    ro, rd = np.array(ro, np.float32), np.array(rd, np.float32)
+    margin = np.array([oracle.ray_relu_margin(t, w, occ, ro[i], rd[i]) for i in range(len(ro))])
+    keep = margin > KINK_MARGIN
+    assert keep.sum() >= 3 * len(cells)
+    ro, rd = ro[keep], rd[keep]
+    target = np.random.default_rng(6).uniform(0, 1, (len(ro), 3)).astype(np.float32)
+    _, gt_ref, gw_ref = oracle.loss_grad(t, w, occ, ro, rd, target)
+    with nes.Trainer(t, w, occ, rays_per_batch=1024) as tr:
+        rgb = tr.forward(ro, rd)
+        gt, gw = tr.backward(ro, rd, 2.0 * (rgb - target) / (3 * len(ro)))
+    scale = np.abs(gt_ref).max()
+    assert np.abs(gt - gt_ref).max() <= 1e-4 * scale
+    assert np.abs(gw - gw_ref).max() <= 1e-4 * np.abs(gw_ref).max()
+    # the straddling entries themselves carry gradient, and it matches
+    hit = 0
+    for lv, x, y, z in cells:
+        n1 = DENSE_RES[lv] + 1
+        e0 = DENSE_OFFSET[lv] + x + (y + z * n1) * n1
+        idx = [2 * e0, 2 * e0 + 1, 2 * e0 + 2, 2 * e0 + 3]  # both features of e0 and e0 + 1
+        if np.abs(gt_ref[idx]).min() > 0:
+            hit += 1
+            assert np.abs(gt[idx] - gt_ref[idx]).max() <= 1e-4 * scale, (lv, x, y, z)
+    assert hit >= len(cells) - 2
+
+
 def poses_and_targets(nes, oracle, scene1, w=640, h=480):
     t, wt, o = scene1
     poses = np.stack([oracle.training_pose(0, i) for i in range(16)])
