@@ -600,6 +600,7 @@ std::vector<SliceTask> slice_tasks() {
   for (;;) {
     auto lo = std::min_element(xcd.begin(), xcd.end(), [&](auto &x, auto &y) { return cost(x) < cost(y); });
     auto hi = std::max_element(xcd.begin(), xcd.end(), [&](auto &x, auto &y) { return cost(x) < cost(y); });
+    if (hi->empty()) break;
     const uint32_t moved = hi->back().level >= (uint32_t)kDenseLevels ? 2u : 1u;
     if (cost(*hi) < cost(*lo) + 2 * moved) break;
     lo->push_back(hi->back());
