@@ -970,8 +970,6 @@ static int trainer_forward(nes_trainer *t, uint32_t n_rays, const TrainRayArgs &
 static int trainer_backward(nes_trainer *t, uint32_t n_rays) {
   RenderWork &w = t->work;
   hipStream_t s = t->stream;
-  const OutLayout L = out_layout(w.out_px);
-  (void)L;
   train_dcomposite<<<(n_rays + 255) / 256, 256, 0, s>>>(
       n_rays, w.rays.as<RayData>(), w.acc.as<RayAcc>(), w.counts.as<uint32_t>(),
       w.offsets.as<uint32_t>(), w.sample_out.as<float4>(), t->dC, t->dout.as<float4>());
