@@ -60,16 +60,16 @@ __device__ __forceinline__ uint32_t ray_box(const float o[3], const float d[3],
   return n;
 }
 
-__device__ __forceinline__ bool occupied(const uint32_t *__restrict__ occ,
-                                         const float x[3]) {
+// Occupancy cell of a position (clamped to the grid, so any position gives
+// an index inside the bitfield).
+__device__ __forceinline__ uint32_t occ_cell(const float x[3]) {
   int c[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     const int ci = (int)floorf(x[a] * 128.0f);
     c[a] = min(max(ci, 0), kOccRes - 1);
   }
-  const uint32_t cell = (uint32_t)(c[0] + kOccRes * (c[1] + kOccRes * c[2]));
-  return (__ldg(occ + (cell >> 5)) >> (cell & 31)) & 1u;
+  return (uint32_t)(c[0] + kOccRes * (c[1] + kOccRes * c[2]));
 }
 
 __device__ __forceinline__ void step_pos(const RayData &r, int k, float x[3],

@@ -119,20 +119,24 @@ __global__ __launch_bounds__(256) void march_emit(MarchArgs m) {
       if (m.resume) k = max(k, (int)(counts[slot] >> 16));
       const int kend = min(m.k_end, (int)r.n_steps);
       knext = (uint32_t)max(kend, k);
-      // four steps' occupancy bits in flight at once, then taken in order
+      // four steps' occupancy words loaded at once (unconditionally: the
+      // clamped cell index is inside the bitfield past kend too), then
+      // taken in order
       constexpr int kStepUnroll = 4;
       bool capped = false;
       for (; k < kend && !capped; k += kStepUnroll) {
-        bool hit[kStepUnroll];
+        uint32_t word[kStepUnroll], cell[kStepUnroll];
 #pragma unroll
         for (int u = 0; u < kStepUnroll; ++u) {
           float x[3], t;
           step_pos(r, k + u, x, &t);
-          hit[u] = k + u < kend && occupied(occ, x);
+          cell[u] = occ_cell(x);
+          word[u] = __ldg(occ + (cell[u] >> 5));
         }
 #pragma unroll
         for (int u = 0; u < kStepUnroll; ++u) {
-          if (hit[u] && !capped) {
+          const bool hit = k + u < kend && ((word[u] >> (cell[u] & 31u)) & 1u);
+          if (hit && !capped) {
             bits[(k + u) >> 5] |= 1u << ((k + u) & 31);
             if (++cnt == m.cap) {
               knext = (uint32_t)(k + u) + 1u;
