@@ -43,6 +43,7 @@ def trainer(nes, scene1):
 
 def test_forward_matches_oracle(oracle, scene1, trainer, rays):
     t, w, o = scene1
+    trainer.set_params(t, w)  # whatever ran before on the shared trainer
     ro, rd, _ = rays
     rgb = trainer.forward(ro, rd)
     ref = np.array([oracle.render_ray(t, w, o, ro[i], rd[i])[0] for i in range(len(ro))])
@@ -51,6 +52,7 @@ def test_forward_matches_oracle(oracle, scene1, trainer, rays):
 
 def test_backward_matches_oracle_gradient(oracle, scene1, trainer, rays):
     t, w, o = scene1
+    trainer.set_params(t, w)
     ro, rd, keep = rays
     ro, rd = ro[keep], rd[keep]
     target = np.random.default_rng(5).uniform(0, 1, (len(ro), 3)).astype(np.float32)
@@ -103,6 +105,32 @@ def test_autograd_binding(nes, oracle, scene1, trainer, rays):
                 vals.append(float(((render_rays(table, wp, trainer, ro, rd) - target) ** 2).mean()))
         fd = (vals[0] - vals[1]) / (2 * h)
         assert abs(fd - gw_ref[i]) <= 0.05 * abs(gw_ref[i]), (i, fd, gw_ref[i])
+
+
+def test_rays_without_samples(oracle, scene1, trainer, rays):
+    """Rays that miss the cube (and one that grazes outside it) evaluate no
+    sample: black, zero gradients everywhere, and the next backward on real
+    rays is unaffected (the bucket sort and slice scatter of an empty step
+    leave nothing behind)."""
+    t, w, o = scene1
+    trainer.set_params(t, w)  # the autograd tests leave perturbed weights behind
+    n = 37
+    ro = np.tile(np.float32([[3.0, 3.0, 3.0]]), (n, 1))
+    rd = np.tile(np.float32([[1.0, 0.0, 0.0]]), (n, 1))  # pointing away from the cube
+    rd[-1] = [0.0, 0.0, -1.0]                              # passes beside it
+    for i in range(n):
+        assert oracle.render_ray(t, w, o, ro[i], rd[i])[3] == 0
+    assert not trainer.forward(ro, rd).any()
+    gt, gw = trainer.backward(ro, rd, np.ones((n, 3), np.float32))
+    assert not gt.any() and not gw.any()
+    ro2, rd2, keep = rays
+    ro2, rd2 = ro2[keep[:50]], rd2[keep[:50]]
+    target = np.random.default_rng(9).uniform(0, 1, (len(ro2), 3)).astype(np.float32)
+    _, gt_ref, gw_ref = oracle.loss_grad(t, w, o, ro2, rd2, target)
+    d_rgb = 2.0 * (trainer.forward(ro2, rd2) - target) / (3 * len(ro2))
+    gt, gw = trainer.backward(ro2, rd2, d_rgb)
+    assert np.abs(gw - gw_ref).max() <= 1e-4 * np.abs(gw_ref).max()
+    assert np.abs(gt - gt_ref).max() <= 1e-4 * np.abs(gt_ref).max()
 
 
 SLICE_ENTRIES = 16384  # entries per LDS slice of the table-gradient scatter
