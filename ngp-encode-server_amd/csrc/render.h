@@ -41,20 +41,20 @@ constexpr uint32_t kPinnedViews = 64;
 // the low 16 bits and the step the next pass resumes at in the high 16 bits.
 constexpr uint32_t kCountMask = 0xffffu;
 
-// Words of RenderWork::total (device) and RenderWork::total_host (pinned).
-//   device: [0] sample cursor of the current pass, [1] samples evaluated by
-//           all passes of the frame (added once per pass), then from word
-//           kTotTermBase kTotTermSlots copies of the termination counters,
-//           one 128-byte line each (a wavefront adds to copy wave % slots,
-//           and only when one of its rays reached T < 1e-4 in the pass):
-//           [+0] rays that reached T < 1e-4, [+1] samples those rays
-//           composited, [+2] samples evaluated behind that point,
-//           [+3] rays that stopped before the last pass with steps left
-//   host:   [0] first-pass sample total, [1] later-pass sample total (0 for
-//           a single pass), [2] device word [1], and from kTotTermBase the
-//           device's counter copies
-constexpr int kTotCursor = 0, kTotEvaluated = 1, kTotTermBase = 32, kTotTermSlots = 32,
-              kTotTermStride = 32;
+// Words of RenderWork::total (device) and RenderWork::total_host (pinned),
+// the same layout on both sides:
+//   [0] sample cursor of the first pass, [1] sample cursor of a later pass
+//   (0 in a single-pass frame), [2] samples evaluated by all passes of the
+//   frame (added once per pass), then from word kTotTermBase kTotTermSlots
+//   copies of the termination counters, one 128-byte line each (a wavefront
+//   adds to copy wave % slots, and only when one of its rays reached
+//   T < 1e-4 in the pass): [+0] rays that reached T < 1e-4, [+1] samples
+//   those rays composited, [+2] samples evaluated behind that point, [+3]
+//   rays that stopped before the last pass with steps left.
+// The host copy of a pass is one transfer: word 0 or 1 after a pass that is
+// not the last, words [0 or 1, kTotWords) after the last.
+constexpr int kTotCursor = 0, kTotCursorLater = 1, kTotEvaluated = 2, kTotTermBase = 32,
+              kTotTermSlots = 32, kTotTermStride = 32;
 constexpr int kTotWords = kTotTermBase + kTotTermSlots * kTotTermStride;
 constexpr size_t kTotBytes = sizeof(uint32_t) * kTotWords;
 
@@ -180,7 +180,7 @@ inline uint32_t part_blocks(size_t capacity) {
 // in work.acc for the backward).
 int launch_march_emit(const uint32_t *occ, RenderWork &w, int k0, int k1, hipStream_t stream);
 int launch_encode_f32(const float *table, RenderWork &w, hipStream_t stream);
-int launch_mlp_f32(const float *weights, RenderWork &w, hipStream_t stream);
+int launch_mlp_f32(const float *weights, RenderWork &w, const uint32_t *n_ptr, hipStream_t stream);
 int launch_composite(RenderWork &w, bool last, hipStream_t stream);
 
 // Enqueue a full render of views on stream; returns after the sample count

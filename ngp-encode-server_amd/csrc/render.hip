@@ -52,7 +52,7 @@ struct MarchArgs {
   RayAcc *acc;
   uint32_t *counts, *offsets, *samples;
   float4 *pos;
-  uint32_t *cursor;  // the frame counters (RenderWork::total)
+  uint32_t *cursor;  // the pass's sample cursor (a word of RenderWork::total)
   uint32_t capacity, cap;
   int k_begin, k_end, first, packed, resume;
   uint2 *ray_sh;
@@ -166,7 +166,7 @@ __global__ __launch_bounds__(256) void march_emit(MarchArgs m) {
   }
   const uint32_t wave_total = wave_sum(cnt);
   uint32_t base = 0;
-  if (lane == 0 && wave_total) base = atomicAdd(m.cursor + kTotCursor, wave_total);
+  if (lane == 0 && wave_total) base = atomicAdd(m.cursor, wave_total);
   base = __shfl(base, 0);
   if (slot < n_slots) {
     counts[slot] = cnt | (knext << 16);
@@ -816,13 +816,16 @@ int render_launch_pass(const SceneDev &scene, RenderWork &w, int precision, int 
                        hipEvent_t ev_eval_end, hipEvent_t ev_encode_end, uint32_t ray_cap,
                        bool resume) {
   const unsigned blocks = (unsigned)((w.n_slots + 255) / 256);
-  uint32_t *cursor = w.total.as<uint32_t>() + kTotCursor;
+  uint32_t *const total = w.total.as<uint32_t>();
+  uint32_t *cursor = total + (first ? kTotCursor : kTotCursorLater);
   const uint32_t cap = w.capacity;
   // a render in one pass keeps no per-ray state between march and composite
   uint2 *tinfo = first && last ? w.tinfo.as<uint2>() : nullptr;
-  // the frame counters start with the first pass, the cursor with every pass
-  zero_u32<<<(kTotWords + 255) / 256, 256, 0, stream>>>(w.total.as<uint32_t>(), first ? kTotWords : 1);
+  // the frame counters start with the first pass, a later pass's cursor with
+  // that pass
+  zero_u32<<<(kTotWords + 255) / 256, 256, 0, stream>>>(first ? total : cursor, first ? kTotWords : 1);
   MarchArgs m = march_args(w, scene.occ, k0, k1, first, ray_cap, resume);
+  m.cursor = cursor;
   m.packed = precision == NES_PRECISION_MIXED ? 1 : 0;
   m.ray_sh = precision == NES_PRECISION_MIXED ? w.ray_sh.as<uint2>() : nullptr;
   m.tinfo = tinfo;
@@ -843,7 +846,7 @@ int render_launch_pass(const SceneDev &scene, RenderWork &w, int precision, int 
                                                  cursor, cap, w.rays.as<RayData>(),
                                                  scene.weights_f32, w.sample_out.as<float4>());
     else
-      NES_TRY(launch_mlp_f32(scene.weights_f32, w, stream));
+      NES_TRY(launch_mlp_f32(scene.weights_f32, w, cursor, stream));
   } else {
     encode_parts<_Float16><<<bpp * kParts, 256, 0, stream>>>(w.sample_pos.as<uint2>(), cursor,
                                                               cap, scene.table_f16,
@@ -870,14 +873,12 @@ int render_launch_pass(const SceneDev &scene, RenderWork &w, int precision, int 
         w.offsets.as<uint32_t>(), w.samples.as<uint32_t>(), w.sample_out.as<uint2>(), cursor, cap,
         0, last ? 1 : 0, optr, tinfo, stats);
   NES_HIP_TRY(hipGetLastError());
-  NES_HIP_TRY(hipMemcpyAsync(w.total_host + (first ? 0 : 1), cursor, 4, hipMemcpyDeviceToHost,
-                             stream));
-  if (last) {
-    NES_HIP_TRY(hipMemcpyAsync(w.total_host + 2, stats + kTotEvaluated, 4, hipMemcpyDeviceToHost,
-                               stream));
-    NES_HIP_TRY(hipMemcpyAsync(w.total_host + kTotTermBase, stats + kTotTermBase,
-                               4 * (kTotWords - kTotTermBase), hipMemcpyDeviceToHost, stream));
-  }
+  // one transfer per pass: its cursor, and after the last pass every word
+  // from there on (a copy costs the same few microseconds at any of these
+  // sizes)
+  const size_t at = (size_t)(cursor - total);
+  NES_HIP_TRY(hipMemcpyAsync(w.total_host + at, cursor, last ? 4 * (kTotWords - at) : 4,
+                             hipMemcpyDeviceToHost, stream));
   return NES_OK;
 }
 

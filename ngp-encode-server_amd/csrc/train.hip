@@ -359,11 +359,12 @@ __global__ __launch_bounds__(256) void mlp_train(TrainMlpArgs a) {
 
 // FP32 forward over the work area's sample list (the FP32 render path and the
 // training forward): density + colour MLPs on v_mfma_f32_16x16x4_f32.
-int nes::launch_mlp_f32(const float *weights, RenderWork &w, hipStream_t stream) {
+int nes::launch_mlp_f32(const float *weights, RenderWork &w, const uint32_t *n_ptr,
+                        hipStream_t stream) {
   TrainMlpArgs ma{};
   ma.pos = w.sample_pos.as<float4>();
   ma.feat = w.feat.as<float4>();
-  ma.n_ptr = w.total.as<uint32_t>();
+  ma.n_ptr = n_ptr;  // the pass's sample count
   ma.stride = w.capacity;
   ma.rays = w.rays.as<RayData>();
   ma.weights = weights;
@@ -987,7 +988,7 @@ static int trainer_forward(nes_trainer *t, uint32_t n_rays, const TrainRayArgs &
       continue;
     }
     NES_TRY(launch_encode_f32(t->params, w, s));
-    NES_TRY(launch_mlp_f32(t->params + t->n_table, w, s));
+    NES_TRY(launch_mlp_f32(t->params + t->n_table, w, w.total.as<uint32_t>(), s));
     NES_TRY(launch_composite(w, true, s));
     return NES_OK;
   }
